@@ -1,0 +1,188 @@
+"""Benchmark of the TRMM hot path on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--n N] [--m M] [--bands P]
+                    [--map reg|balanced] [--shape boxes|trapezoid] [--no-cpu]
+
+A step is one C = tril(A) * B (FP64) over the whole matrix, with A, B and C
+resident in HBM before the timed region.  Default workload: N = 32768,
+M = 8192, balanced mapping over 16 bands, trapezoid tile walk, inputs
+U[-1,1) with exponents spread over 2^[-8,8] (seeded generator, on device).
+
+Prints one JSON line.  value = M*N^2 / (mean step time) in TFLOP/s.
+roofline: the dominant kernel (the tile kernel) against FP64 MFMA peak,
+timed with HIP events on the stream the kernel runs on.
+cpu_baseline: the reference's per-rank stage loop (cblas_dgemm + cblas_dtrmm
+with MKL, sequential, one thread per rank) replayed on the host cores for a
+bounded column sample of the same matrices; its C columns double as a parity
+check of the GPU result.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(args, C_gpu_cols, cols_needed):
+    """Reference stage loop on host cores (MKL) for a column sample."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from conftest import Oracle  # the oracle library: CPU comparator and checker
+
+    orc = Oracle()
+    ver = ctypes.create_string_buffer(256)
+    have_mkl = bool(orc.L.orc_blas_available(ver, 256))
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    ranks = max(1, min(avail, int(os.environ.get("OMP_NUM_THREADS", avail)), 16))
+    n, m = args.n, args.m
+    w = args.cpu_cols_per_rank if have_mkl else max(2, args.cpu_cols_per_rank // 8)
+    w = min(w, m // ranks)
+    log(f"cpu baseline: {ranks} ranks x {w} columns, mkl={have_mkl}")
+    t0 = time.time()
+    A = np.empty((n, n))
+    orc.L.orc_fill_seeded_A(n, args.seed, args.spread, orc._p(A), n)
+    B = np.empty((n, m))
+    orc.L.orc_fill_seeded_B(n, m, args.seed, args.spread, orc._p(B), m)
+    Bs = np.ascontiguousarray(B[:, : ranks * w])
+    del B
+    Cs = np.zeros((n, ranks * w))
+    log(f"host inputs ready in {time.time() - t0:.1f}s")
+    secs = orc.L.orc_ranks_stage_loop(n, w, args.bands, 1 if args.map == "balanced" else 0, ranks,
+                                      orc._p(A), n, orc._p(Bs), ranks * w, orc._p(Cs), ranks * w,
+                                      1 if have_mkl else 0)
+    flops = float(n) * n * (ranks * w)
+    err = None
+    if C_gpu_cols is not None:
+        G = C_gpu_cols[:, : ranks * w]
+        err = float(np.linalg.norm(G - Cs) / np.linalg.norm(Cs))
+    return {
+        "value": flops / secs * 1e-12,
+        "unit": "TFLOP/s",
+        "cores": ranks,
+        "kind": "port",
+        "sample": (f"reference stage loop (dgemm box + dtrmm triangle per band, "
+                   f"{args.map} {args.bands} bands) for {ranks} ranks x {w} columns of the "
+                   f"N={n} workload ({ranks * w} of {m} columns), "
+                   f"{'MKL ' + ver.value.decode()[:40] + ' sequential' if have_mkl else 'C loops'}, "
+                   f"{secs:.2f} s"),
+        "seconds": secs,
+        "gpu_vs_cpu_rel_fro": err,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=32768)
+    ap.add_argument("--m", type=int, default=8192)
+    ap.add_argument("--bands", type=int, default=16)
+    ap.add_argument("--map", default="balanced", choices=["reg", "balanced"])
+    ap.add_argument("--shape", default="trapezoid", choices=["boxes", "trapezoid"])
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--spread", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-cols-per-rank", type=int, default=96)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the TRMM path shards by columns of B with no "
+              "reduction, and no multi-GPU run is part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import uvatrmm_amd as uvt
+
+    n, m = args.n, args.m
+    h = uvt.Handle(0)
+    log(f"device cus={h.cus}; allocating N={n} M={m}")
+    dA, dB, dC = h.alloc(n * n * 8), h.alloc(n * m * 8), h.alloc(n * m * 8)
+    h.fill_seeded(n, m, args.seed, args.spread, dA, n, dB, m)
+    plan = uvt.Plan(n, m, args.bands, args.map, args.shape)
+    flops = plan.flops
+    h.sync()
+    log(f"plan: {plan.phases} launch(es), tiles={[len(plan.tiles(p)) for p in range(plan.phases)]}, "
+        f"issued/algorithmic={plan.issued_flops / flops:.4f}")
+    for _ in range(args.warmup):
+        h.trmm_f64(plan, dA, n, dB, m, dC, m)
+    h.sync()
+    evs = [(h.event(), h.event()) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for e0, e1 in evs:
+        h.record(e0)
+        h.trmm_f64(plan, dA, n, dB, m, dC, m)
+        h.record(e1)
+    h.sync()
+    wall = time.perf_counter() - t0
+    step_ms = [h.elapsed_ms(e0, e1) for e0, e1 in evs]
+    mean_ms = float(np.mean(step_ms))
+    value = flops / (wall / args.steps) * 1e-12
+    achieved = flops / (mean_ms * 1e-3) * 1e-12
+    log(f"steps ms: min {min(step_ms):.2f} mean {mean_ms:.2f} max {max(step_ms):.2f}; "
+        f"wall/step {wall / args.steps * 1e3:.2f} ms; {achieved:.2f} TFLOP/s")
+    norm = h.inf_norm(n, m, dC, m)
+
+    cpu = None
+    if not args.no_cpu:
+        cols = min(m, 16 * args.cpu_cols_per_rank)
+        Cg = np.empty((n, cols))
+        full = dC.download((n, m)) if m * n * 8 <= (1 << 34) else None
+        if full is not None:
+            Cg = np.ascontiguousarray(full[:, :cols])
+            del full
+        else:
+            Cg = None
+        try:
+            cpu = cpu_baseline(args, Cg, cols)
+            log(f"cpu baseline {cpu['value']:.4f} TFLOP/s on {cpu['cores']} cores; "
+                f"gpu vs cpu rel_fro {cpu['gpu_vs_cpu_rel_fro']}")
+        except Exception as e:  # the baseline is reported, never the measured path
+            log("cpu baseline failed:", repr(e))
+
+    kernel = "k_trmm_tiles_f64<128,128,VEC>"
+    out = {
+        "metric": "TRMM TFLOP/s at N=32768 (plain FP64, % of FP64 MFMA peak)",
+        "value": round(value, 4),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded U[-1,1) x 2^[-8,8], generated on device)",
+        "config": {"workload": f"trmm_f64 left lower non-unit N={n} M={m} {args.map} "
+                               f"{args.shape} P={args.bands}",
+                   "n": n, "m": m, "bands": args.bands, "map": args.map, "shape": args.shape},
+        "roofline": {"bound": "mfma", "kernel": kernel, "achieved": round(achieved, 4),
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                     "launches_per_step": plan.phases,
+                     "algorithmic_flops_per_step": flops},
+        "cpu_baseline": cpu,
+        "norm": norm,
+    }
+    print(json.dumps(out), flush=True)
+    for b in (dA, dB, dC):
+        b.free()
+    plan.close()
+    h.close()
+
+
+if __name__ == "__main__":
+    main()

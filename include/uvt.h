@@ -1,0 +1,175 @@
+/*
+ * uvt.h -- C ABI of the MI355X TRMM engine (libuvt.so).
+ *
+ * The engine computes the local product of the UVaTRMM program,
+ * C = tril(A) * B for a lower-triangular, non-unit, row-major FP64 A (n x n)
+ * and row-major B, C (n x m), on one AMD MI355X (gfx950).
+ *
+ * What each group of entry points replaces in the reference (paths relative
+ * to the UVaTRMM source tree):
+ *   uvt_map_band / uvt_plan_bands   map_regular / map_balanced,
+ *                                   UVaTRMM/src/mm_mapping.c:27-67, 319-366
+ *   uvt_plan_*                      the stage loop's per-band shapes,
+ *                                   UVaTRMM/src/mm_triangular.c:319-410
+ *                                   (and the boxes/trapezoid shape choice,
+ *                                   UVaTRMM/src/mm_triangular_common.c:1477-1513)
+ *   uvt_trmm_f64                    the per-rank cblas_dgemm + cblas_dtrmm +
+ *                                   C += B sequence, mm_triangular.c:347-393
+ *   uvt_inf_norm / uvt_row_abs_sums matrix_rangeInfinityNorm,
+ *                                   mm_triangular_common.c:2936-2991
+ *   uvt_fill_closed_form            mat_consecutive_values (MININUMS=1),
+ *                                   mm_triangular_common.c:1782-1857
+ *   uvt_trmm_split, uvt_batch_*,    no counterpart in the reference (split
+ *   uvt_serve_*                     precision, batched and served TRMMs)
+ *
+ * Conventions: every pointer named d* is device memory owned by the caller;
+ * sizes and leading dimensions are element counts (int64); every function
+ * returns a uvt_status and never aborts the process; work is asynchronous on
+ * the given stream (a hipStream_t passed as void*, NULL = the null stream)
+ * unless a comment says "synchronous".  A handle is not thread-safe: use one
+ * per host thread.
+ */
+#ifndef UVT_H
+#define UVT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    UVT_OK = 0,
+    UVT_ERR_ARG = 1,         /* null pointer, negative size, bad enum, ld too small */
+    UVT_ERR_SIZE = 2,        /* a size the engine does not support */
+    UVT_ERR_MAPPING = 3,     /* the reference would abort on this mapping (see ref_exit) */
+    UVT_ERR_HIP = 4,         /* a HIP runtime call failed */
+    UVT_ERR_NOMEM = 5,       /* host or device allocation failed */
+    UVT_ERR_BUDGET = 6,      /* request exceeds the serving memory budget */
+    UVT_ERR_UNSUPPORTED = 7, /* no GPU, or a feature this build lacks */
+    UVT_ERR_STATE = 8,       /* call out of order (e.g. server closed) */
+    UVT_ERR_AGAIN = 9        /* nothing ready yet (polling) */
+} uvt_status;
+
+/* Range{begin,size} of include/mkl_range.h:18-21 (rows of a band). */
+typedef struct {
+    uint64_t begin;
+    uint64_t size;
+} uvt_range;
+
+typedef enum { UVT_MAP_REG = 0, UVT_MAP_BALANCED = 1 } uvt_map;
+/* boxes: per band a rectangular GEMM box over k in [0, b] followed by the
+ * (h-1)-row diagonal triangle accumulated into C (two launches).
+ * trapezoid: one triangle-aware tile walk over the lower trapezoid. */
+typedef enum { UVT_SHAPE_BOXES = 0, UVT_SHAPE_TRAPEZOID = 1 } uvt_shape;
+typedef enum { UVT_PREC_F64 = 0, UVT_PREC_SPLIT = 1 } uvt_prec;
+
+const char *uvt_status_string(uvt_status s);
+const char *uvt_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Band mapping (host only; no GPU needed).                            */
+/* ------------------------------------------------------------------ */
+/* Band i of p over n rows, bit-identical to the reference.  When the
+ * reference would abort, returns UVT_ERR_MAPPING and sets *ref_exit to the
+ * reference's process exit code (7: too many bands, 11: an empty band);
+ * n whose balanced element count overflows the reference's 32-bit counter
+ * (n >= 92682) give UVT_ERR_SIZE.  ref_exit may be NULL. */
+uvt_status uvt_map_band(uint64_t n, uint32_t p, uint32_t i, uvt_map map, uvt_range *out,
+                        int *ref_exit);
+/* All p bands (out has room for p). */
+uvt_status uvt_plan_bands(uint64_t n, uint32_t p, uvt_map map, uvt_range *out, int *ref_exit);
+
+/* ------------------------------------------------------------------ */
+/* Tile plans (host objects; uploaded to the device on first use).     */
+/* ------------------------------------------------------------------ */
+typedef struct uvt_plan_s *uvt_plan;
+
+/* One output tile of the FP64 kernels: rows [r0, r0+rows), columns
+ * [c0, c0+tile_n) clipped to m, reduction over k in [k_lo, k_hi). */
+typedef struct {
+    int32_t r0, rows, c0, k_lo, k_hi, flags, band, cfg;
+} uvt_tile;
+#define UVT_TILE_LOWER 1 /* A[i][k] is treated as 0 for k > i */
+#define UVT_TILE_ACCUM 2 /* C += product (else C = product) */
+
+/* Plans C = tril(A) * B for an n x n A and n x m B over p bands of the given
+ * mapping and shape.  Errors as uvt_plan_bands. */
+uvt_status uvt_plan_create(int64_t n, int64_t m, uint32_t p, uvt_map map, uvt_shape shape,
+                           uvt_plan *out, int *ref_exit);
+uvt_status uvt_plan_destroy(uvt_plan plan);
+/* Number of launches (phases): 2 for boxes, 1 for trapezoid. */
+int32_t uvt_plan_phases(uvt_plan plan);
+int64_t uvt_plan_num_tiles(uvt_plan plan, int32_t phase);
+uvt_status uvt_plan_get_tiles(uvt_plan plan, int32_t phase, uvt_tile *out, int64_t cap);
+uvt_status uvt_plan_get_bands(uvt_plan plan, uvt_range *out, uint32_t cap);
+/* Tile edge lengths used by the plan (rows, columns). */
+uvt_status uvt_plan_tile_shape(uvt_plan plan, int32_t *tile_m, int32_t *tile_n);
+/* Algorithmic flops m*n^2 (LAWN 41 count for a left triangular product). */
+double uvt_plan_flops(uvt_plan plan);
+/* MFMA flops actually issued, including masked diagonal tiles and padding. */
+double uvt_plan_issued_flops(uvt_plan plan);
+
+/* ------------------------------------------------------------------ */
+/* Device handle.                                                      */
+/* ------------------------------------------------------------------ */
+typedef struct uvt_handle_s *uvt_handle;
+uvt_status uvt_create(int device, uvt_handle *out);
+uvt_status uvt_destroy(uvt_handle h);
+/* Number of compute units of the handle's device. */
+int32_t uvt_device_cus(uvt_handle h);
+
+/* FP64 C = tril(A) * B following the plan (out of place; A and B are not
+ * written; C is fully overwritten on rows [0, n) x columns [0, m)). */
+uvt_status uvt_trmm_f64(uvt_handle h, uvt_plan plan, const double *dA, int64_t lda,
+                        const double *dB, int64_t ldb, double *dC, int64_t ldc, void *stream);
+
+/* Split-precision C ~= tril(A) * B: the off-diagonal bulk as exact int8
+ * slice products (int32 accumulation) recombined in FP64; the diagonal
+ * blocks in FP64.  `slices` = number of int8 slices per operand (2..8);
+ * more slices, smaller error.  workspace: see uvt_trmm_split_workspace. */
+uvt_status uvt_trmm_split_workspace(int64_t n, int64_t m, int32_t slices, size_t *bytes);
+uvt_status uvt_trmm_split(uvt_handle h, int64_t n, int64_t m, int32_t slices, const double *dA,
+                          int64_t lda, const double *dB, int64_t ldb, double *dC, int64_t ldc,
+                          void *dwork, size_t work_bytes, void *stream);
+
+/* Row sums of |C| into d_rowsum[n] (device). */
+uvt_status uvt_row_abs_sums(uvt_handle h, int64_t n, int64_t m, const double *dC, int64_t ldc,
+                            double *d_rowsum, void *stream);
+/* max_i sum_j |C[i][j]|; synchronous, result in *norm (host). */
+uvt_status uvt_inf_norm(uvt_handle h, int64_t n, int64_t m, const double *dC, int64_t ldc,
+                        double *norm, void *stream);
+
+/* Inputs.  Closed form of the reference (A lower with zeros above the
+ * diagonal, B n x m with value index row*n + column). */
+uvt_status uvt_fill_closed_form(uvt_handle h, int64_t n, int64_t m, double *dA, int64_t lda,
+                                double *dB, int64_t ldb, void *stream);
+/* Seeded inputs: SplitMix64 counter generator, U[-1,1), A's diagonal +2.0,
+ * spread!=0 scales each entry by 2^e, e uniform in {-8..8}.  Bit-identical
+ * to the host generator of the same name. Either pointer may be NULL. */
+uvt_status uvt_fill_seeded(uvt_handle h, int64_t n, int64_t m, uint64_t seed, int32_t spread,
+                           double *dA, int64_t lda, double *dB, int64_t ldb, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Small runtime helpers for callers without a HIP toolchain (ctypes,  */
+/* the CLI).                                                           */
+/* ------------------------------------------------------------------ */
+uvt_status uvt_malloc(uvt_handle h, size_t bytes, void **dptr);
+uvt_status uvt_free(uvt_handle h, void *dptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device; synchronous. */
+uvt_status uvt_memcpy(uvt_handle h, void *dst, const void *src, size_t bytes, int32_t kind);
+uvt_status uvt_memset(uvt_handle h, void *dptr, int32_t value, size_t bytes);
+uvt_status uvt_stream_create(uvt_handle h, void **stream);
+uvt_status uvt_stream_destroy(uvt_handle h, void *stream);
+uvt_status uvt_stream_sync(uvt_handle h, void *stream);
+uvt_status uvt_event_create(uvt_handle h, void **event);
+uvt_status uvt_event_destroy(uvt_handle h, void *event);
+uvt_status uvt_event_record(uvt_handle h, void *event, void *stream);
+/* Milliseconds between two recorded events (synchronises on `end`). */
+uvt_status uvt_event_elapsed(uvt_handle h, void *start, void *end, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

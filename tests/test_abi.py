@@ -1,0 +1,55 @@
+"""The C ABI without a GPU: the library loads, exports every declared entry
+point, and the command line keeps the reference's argument errors."""
+import ctypes
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+from conftest import ROOT
+
+HEADERS = ["uvt.h"]
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(uvt_\w+|cblas_\w+)\s*\(", text)
+    return sorted(set(n for n in names if not n.endswith("_s")))
+
+
+@pytest.mark.parametrize("header", HEADERS)
+def test_library_exports_declared_symbols(uvt, header):
+    lib = ctypes.CDLL(uvt.LIB_PATH if header != "uvt_cblas.h" else uvt.CBLAS_PATH)
+    names = declared(header)
+    assert len(names) > 10 or header != "uvt.h"
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_version_and_status_strings(uvt):
+    assert b"gfx950" in uvt.lib().uvt_version()
+    assert uvt.lib().uvt_status_string(uvt.ERR_MAPPING).startswith(b"mapping")
+
+
+def _cli(*args):
+    return subprocess.run([os.path.join(ROOT, "uvatrmm_amd", "bin", "uvatrmm.x"), *args],
+                          capture_output=True, text=True, timeout=60, cwd=tempfile.mkdtemp())
+
+
+def test_cli_usage_errors_exit_3(uvt):
+    for args, msg in ((["2048", "reg"], "Number of arguments"),
+                      (["2048", "foo", "boxes"], "Unknown name of mapping function"),
+                      (["2048", "reg", "full"], "Unknown name of communication shape"),
+                      (["3", "reg", "boxes", "--bands", "4"],
+                       "Number of rows should be equal or greater than number of processors")):
+        r = _cli(*args)
+        assert r.returncode == 3, (args, r.returncode)
+        assert f"Error: {msg}" in r.stderr
+        assert "Usage:" in r.stderr
+
+
+def test_cli_mapping_abort_codes(uvt):
+    assert _cli("7", "balanced", "boxes", "--bands", "7").returncode == 11

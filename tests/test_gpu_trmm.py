@@ -1,0 +1,233 @@
+"""FP64 TRMM on the MI355X through libuvt's C ABI, checked against the oracle.
+
+Tolerance: relative Frobenius error <= 1e-13 against the CPU oracle (the
+bar stated for the plain FP64 path); exact equality where the arithmetic is
+exact (identity / dyadic inputs, generators, repeat runs).
+"""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, rel_fro
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-13
+
+
+def say(*a):
+    print("[gpu]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def run_trmm(gpu, uvt, A, B, p=1, mapping="reg", shape="trapezoid", lda=None):
+    n, m = B.shape
+    lda = lda or A.shape[1]
+    say("trmm_f64", dict(n=n, m=m, p=p, map=mapping, shape=shape, lda=lda))
+    plan = uvt.Plan(n, m, p, mapping, shape)
+    dA, dB, dC = gpu.array(A), gpu.array(B), gpu.alloc(n * m * 8)
+    try:
+        gpu.trmm_f64(plan, dA, lda, dB, m, dC, m)
+        gpu.sync()
+        return dC.download((n, m))
+    finally:
+        for b in (dA, dB, dC):
+            b.free()
+        plan.close()
+
+
+def test_small_exact_cases(gpu, uvt):
+    """Identity and all-ones lower A give exact results: checks the MFMA
+    operand/result layouts and every tile edge."""
+    for n, m in ((1, 1), (5, 3), (16, 16), (17, 33), (130, 129), (256, 300)):
+        B = (np.arange(n * m, dtype=np.float64).reshape(n, m) % 7) - 3.0
+        for shape, p in (("trapezoid", 1), ("boxes", min(3, n))):
+            C = run_trmm(gpu, uvt, np.eye(n), B, p=p, shape=shape)
+            np.testing.assert_array_equal(C, B)
+            C = run_trmm(gpu, uvt, np.tril(np.ones((n, n))), B, p=p, shape=shape)
+            np.testing.assert_array_equal(C, np.cumsum(B, axis=0))
+
+
+CASES = [
+    (2048, 2048, 4, "reg", "boxes"),  # the CPU-runnable reference configuration
+    (2048, 2048, 4, "reg", "trapezoid"),
+    (1000, 777, 3, "balanced", "boxes"),
+    (513, 129, 16, "balanced", "trapezoid"),
+    (127, 1, 2, "reg", "boxes"),
+    (64, 300, 1, "reg", "trapezoid"),
+    (4097, 160, 8, "balanced", "boxes"),
+    (3000, 250, 7, "balanced", "trapezoid"),
+]
+
+
+@pytest.mark.parametrize("n,m,p,mp,shape", CASES)
+def test_parity_seeded(gpu, uvt, oracle, n, m, p, mp, shape):
+    A = oracle.seeded_A(n, 100 + n, 0)
+    B = oracle.seeded_B(n, m, 100 + n, 0)
+    C = run_trmm(gpu, uvt, A, B, p, mp, shape)
+    err = rel_fro(C, oracle.trmm(A, B))
+    say("rel_fro", err)
+    assert err <= TOL
+
+
+def test_parity_exponent_spread(gpu, uvt, oracle):
+    n, m = 1500, 300
+    A = oracle.seeded_A(n, 7, 1)
+    B = oracle.seeded_B(n, m, 7, 1)
+    C = run_trmm(gpu, uvt, A, B, 5, "balanced", "trapezoid")
+    assert rel_fro(C, oracle.trmm(A, B)) <= TOL
+
+
+def test_mkl_trmm_fixtures(gpu, uvt, oracle):
+    import re
+    d = np.load(os.path.join(GOLDEN, "mkl_trmm.npz"))
+    for k in d.files:
+        mt = re.fullmatch(r"s(\d+)_n(\d+)_m(\d+)_sp(\d+)_C", k)
+        if not mt:
+            continue
+        seed, n, m, sp = map(int, mt.groups())
+        A, B = oracle.seeded_A(n, seed, sp), oracle.seeded_B(n, m, seed, sp)
+        for shape in ("boxes", "trapezoid"):
+            C = run_trmm(gpu, uvt, A, B, min(4, n), "balanced" if n > 4 else "reg", shape)
+            assert rel_fro(C, d[k]) <= TOL, (k, shape)
+
+
+def test_mkl_stage_loop_fixtures(gpu, uvt, oracle):
+    """The reference's own per-rank dgemm+dtrmm sequence (run with MKL) on its
+    closed-form inputs, same band count and mapping, both shapes."""
+    d = np.load(os.path.join(GOLDEN, "mkl_stage_loop.npz"))
+    for k in d.files:
+        if not k.endswith("_C"):
+            continue
+        n, p, mp, _ = k.split("_")
+        n, p = int(n[1:]), int(p[1:])
+        A, B = oracle.closed_A(n), oracle.closed_B(n, n)
+        for shape in ("boxes", "trapezoid"):
+            C = run_trmm(gpu, uvt, A, B, p, mp, shape)
+            assert rel_fro(C, d[k]) <= TOL, (k, shape)
+
+
+def test_generators_bit_exact(gpu, uvt, oracle):
+    n, m = 333, 517
+    dA, dB = gpu.alloc(n * n * 8), gpu.alloc(n * m * 8)
+    say("fill closed form", n, m)
+    gpu.fill_closed_form(n, m, dA, n, dB, m)
+    gpu.sync()
+    np.testing.assert_array_equal(dA.download((n, n)), oracle.closed_A(n))
+    np.testing.assert_array_equal(dB.download((n, m)), oracle.closed_B(n, m))
+    for spread in (0, 1):
+        say("fill seeded", spread)
+        gpu.fill_seeded(n, m, 42, spread, dA, n, dB, m)
+        gpu.sync()
+        np.testing.assert_array_equal(dA.download((n, n)), oracle.seeded_A(n, 42, spread))
+        np.testing.assert_array_equal(dB.download((n, m)), oracle.seeded_B(n, m, 42, spread))
+    dA.free()
+    dB.free()
+
+
+@pytest.mark.parametrize("n", [257, 1000, 2048, 4096])
+def test_reference_norm_on_gpu(gpu, uvt, n):
+    """Closed-form inputs generated on the GPU, product, norm: the reference's Norm line."""
+    with open(os.path.join(GOLDEN, "reference_norms.json")) as f:
+        want = {d["n"]: d["norm"] for d in json.load(f)["norms"]}[n]
+    dA, dB, dC = gpu.alloc(n * n * 8), gpu.alloc(n * n * 8), gpu.alloc(n * n * 8)
+    say("norm case", n)
+    gpu.fill_closed_form(n, n, dA, n, dB, n)
+    for mp, shape, p in (("reg", "boxes", 4), ("balanced", "trapezoid", 8)):
+        plan = uvt.Plan(n, n, p, mp, shape)
+        gpu.trmm_f64(plan, dA, n, dB, n, dC, n)
+        assert "%.3f" % gpu.inf_norm(n, n, dC, n) == want
+        plan.close()
+    for b in (dA, dB, dC):
+        b.free()
+
+
+def test_inf_norm_and_row_sums(gpu, uvt, oracle):
+    C = oracle.seeded_B(1000, 3001, 5, 1)
+    dC = gpu.array(C)
+    v, rs = oracle.inf_norm(C)
+    assert abs(gpu.inf_norm(1000, 3001, dC, 3001) - v) <= 1e-14 * v
+    out = gpu.alloc(1000 * 8)
+    gpu.row_abs_sums(1000, 3001, dC, 3001, out)
+    gpu.sync()
+    np.testing.assert_allclose(out.download(1000), rs, rtol=1e-14)
+    dC.free()
+    out.free()
+
+
+def test_odd_leading_dimension_and_upper_garbage(gpu, uvt, oracle):
+    """lda odd (unaligned rows: scalar-load path) and junk above the diagonal."""
+    n, m = 700, 90
+    A = oracle.seeded_A(n, 3)
+    B = oracle.seeded_B(n, m, 3)
+    Abig = np.full((n, n + 1), np.nan)
+    Abig[:, :n] = A + np.triu(np.full((n, n), 1e300), 1)
+    ref = oracle.trmm(A, B)
+    for shape in ("boxes", "trapezoid"):
+        C = run_trmm(gpu, uvt, Abig, B, 3, "reg", shape, lda=n + 1)
+        assert rel_fro(C, ref) <= TOL
+
+
+def test_boxes_equals_trapezoid_and_repeatable(gpu, uvt, oracle):
+    n, m = 2500, 300
+    A, B = oracle.seeded_A(n, 9), oracle.seeded_B(n, m, 9)
+    c1 = run_trmm(gpu, uvt, A, B, 6, "balanced", "boxes")
+    c2 = run_trmm(gpu, uvt, A, B, 6, "balanced", "trapezoid")
+    c3 = run_trmm(gpu, uvt, A, B, 6, "balanced", "trapezoid")
+    assert rel_fro(c1, c2) <= 1e-14
+    np.testing.assert_array_equal(c2, c3)
+
+
+def test_large_row_sampled(gpu, uvt, oracle):
+    """N=16384 balanced/trapezoid over 16 bands (the single-GPU reference
+    configuration's shape) with B cut to 512 columns; rows checked against
+    double-double sums."""
+    n, m = 16384, 512
+    dA, dB, dC = gpu.alloc(n * n * 8), gpu.alloc(n * m * 8), gpu.alloc(n * m * 8)
+    say("large case", n, m)
+    gpu.fill_seeded(n, m, 77, 0, dA, n, dB, m)
+    plan = uvt.Plan(n, m, 16, "balanced", "trapezoid")
+    gpu.trmm_f64(plan, dA, n, dB, m, dC, m)
+    gpu.sync()
+    rows = np.array([0, 1, 4094, 4095, 4096, 8191, 12000, 15862, 15863, 16383])
+    A = oracle.seeded_A(n, 77)
+    B = oracle.seeded_B(n, m, 77)
+    ref = oracle.trmm_rows_dd(A, B, rows)
+    C = dC.download((n, m))
+    for r, i in enumerate(rows):
+        assert rel_fro(C[i], ref[r]) <= TOL, i
+    # the rest through a size-independent property: linearity in B
+    dB2, dC2 = gpu.alloc(n * m * 8), gpu.alloc(n * m * 8)
+    gpu.fill_seeded(n, m, 78, 0, None, n, dB2, m)
+    B2 = dB2.download((n, m))
+    dB.upload(B + B2)
+    gpu.trmm_f64(plan, dA, n, dB, m, dC2, m)
+    gpu.trmm_f64(plan, dA, n, dB2, m, dB, m)
+    gpu.sync()
+    lhs, rhs = dC2.download((n, m)), C + dB.download((n, m))
+    assert rel_fro(lhs, rhs) <= 1e-13
+    plan.close()
+    for b in (dA, dB, dC, dB2, dC2):
+        b.free()
+
+
+def test_cli_reference_output(uvt):
+    """uvatrmm.x 2048 reg boxes with 4 bands prints the reference's lines."""
+    say("cli 2048 reg boxes --bands 4")
+    import tempfile
+    cwd = tempfile.mkdtemp()
+    out = subprocess.run([uvt.CLI_PATH, "2048", "reg", "boxes", "--bands", "4"], capture_output=True,
+                         text=True, timeout=120, cwd=cwd)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.split("\n")
+    assert lines[0] == "" and lines[1].startswith("Total t: ")
+    assert lines[2] == "Total mpis: 4" and lines[3].startswith("Mult t: ")
+    assert "\nNorm: 61.447\n" in out.stdout
+    with open(os.path.join(cwd, "norm_ff.txt")) as f:
+        assert f.read() == "61.447"
+    out = subprocess.run([uvt.CLI_PATH, "1000", "balanced", "trapezoid", "--bands", "3"],
+                         capture_output=True, text=True, timeout=120, cwd=cwd)
+    assert "\nNorm: 29.286\n" in out.stdout

@@ -1,0 +1,305 @@
+"""Python bindings (ctypes) for libuvt, the MI355X TRMM engine.
+
+The engine itself is C + HIP (``uvatrmm_amd/csrc``, ABI in ``include/uvt.h``).
+This module only loads the in-tree ``lib/libuvt.so`` and wraps its entry
+points for the test suite and the benchmark; every computation runs in the
+library.  If the library is missing, :func:`lib` raises -- there is no
+Python or CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libuvt.so")
+CBLAS_PATH = os.path.join(HERE, "lib", "libuvt_cblas.so")
+CLI_PATH = os.path.join(HERE, "bin", "uvatrmm.x")
+
+OK, ERR_ARG, ERR_SIZE, ERR_MAPPING, ERR_HIP, ERR_NOMEM, ERR_BUDGET, ERR_UNSUPPORTED, ERR_STATE, \
+    ERR_AGAIN = range(10)
+MAP_REG, MAP_BALANCED = 0, 1
+SHAPE_BOXES, SHAPE_TRAPEZOID = 0, 1
+TILE_LOWER, TILE_ACCUM = 1, 2
+MAPS = {"reg": MAP_REG, "balanced": MAP_BALANCED}
+SHAPES = {"boxes": SHAPE_BOXES, "trapezoid": SHAPE_TRAPEZOID}
+
+_c = ctypes
+i32, i64, u32, u64, f64 = _c.c_int32, _c.c_int64, _c.c_uint32, _c.c_uint64, _c.c_double
+vp, sz = _c.c_void_p, _c.c_size_t
+
+
+class Range(_c.Structure):
+    _fields_ = [("begin", u64), ("size", u64)]
+
+
+class Tile(_c.Structure):
+    _fields_ = [(k, i32) for k in ("r0", "rows", "c0", "k_lo", "k_hi", "flags", "band", "cfg")]
+
+
+class UvtError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        msg = lib().uvt_status_string(status).decode() if _LIB else str(status)
+        super().__init__(f"{what}: {msg} (status {status})" if what else msg)
+
+
+_LIB = None
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "uvt_status_string": (_c.c_char_p, [i32]),
+    "uvt_version": (_c.c_char_p, []),
+    "uvt_map_band": (i32, [u64, u32, u32, i32, _c.POINTER(Range), _c.POINTER(_c.c_int)]),
+    "uvt_plan_bands": (i32, [u64, u32, i32, _c.POINTER(Range), _c.POINTER(_c.c_int)]),
+    "uvt_plan_create": (i32, [i64, i64, u32, i32, i32, _c.POINTER(vp), _c.POINTER(_c.c_int)]),
+    "uvt_plan_destroy": (i32, [vp]),
+    "uvt_plan_phases": (i32, [vp]),
+    "uvt_plan_num_tiles": (i64, [vp, i32]),
+    "uvt_plan_get_tiles": (i32, [vp, i32, _c.POINTER(Tile), i64]),
+    "uvt_plan_get_bands": (i32, [vp, _c.POINTER(Range), u32]),
+    "uvt_plan_tile_shape": (i32, [vp, _c.POINTER(i32), _c.POINTER(i32)]),
+    "uvt_plan_flops": (f64, [vp]),
+    "uvt_plan_issued_flops": (f64, [vp]),
+    "uvt_create": (i32, [_c.c_int, _c.POINTER(vp)]),
+    "uvt_destroy": (i32, [vp]),
+    "uvt_device_cus": (i32, [vp]),
+    "uvt_trmm_f64": (i32, [vp, vp, vp, i64, vp, i64, vp, i64, vp]),
+    "uvt_trmm_split_workspace": (i32, [i64, i64, i32, _c.POINTER(sz)]),
+    "uvt_trmm_split": (i32, [vp, i64, i64, i32, vp, i64, vp, i64, vp, i64, vp, sz, vp]),
+    "uvt_row_abs_sums": (i32, [vp, i64, i64, vp, i64, vp, vp]),
+    "uvt_inf_norm": (i32, [vp, i64, i64, vp, i64, _c.POINTER(f64), vp]),
+    "uvt_fill_closed_form": (i32, [vp, i64, i64, vp, i64, vp, i64, vp]),
+    "uvt_fill_seeded": (i32, [vp, i64, i64, u64, i32, vp, i64, vp, i64, vp]),
+    "uvt_malloc": (i32, [vp, sz, _c.POINTER(vp)]),
+    "uvt_free": (i32, [vp, vp]),
+    "uvt_memcpy": (i32, [vp, vp, vp, sz, i32]),
+    "uvt_memset": (i32, [vp, vp, i32, sz]),
+    "uvt_stream_create": (i32, [vp, _c.POINTER(vp)]),
+    "uvt_stream_destroy": (i32, [vp, vp]),
+    "uvt_stream_sync": (i32, [vp, vp]),
+    "uvt_event_create": (i32, [vp, _c.POINTER(vp)]),
+    "uvt_event_destroy": (i32, [vp, vp]),
+    "uvt_event_record": (i32, [vp, vp, vp]),
+    "uvt_event_elapsed": (i32, [vp, vp, vp, _c.POINTER(_c.c_float)]),
+}
+
+
+def lib():
+    """The loaded libuvt.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                               "or make -C uvatrmm_amd")
+        L = _c.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(status, what=""):
+    if status != OK:
+        raise UvtError(status, what)
+
+
+# ------------------------------------------------------------------ mapping
+def map_band(n, p, i, mapping):
+    """Band i of p (begin, size) or raises UvtError (with .ref_exit)."""
+    r = Range()
+    ex = _c.c_int(0)
+    st = lib().uvt_map_band(n, p, i, MAPS.get(mapping, mapping), _c.byref(r), _c.byref(ex))
+    if st != OK:
+        e = UvtError(st, "map_band")
+        e.ref_exit = ex.value
+        raise e
+    return r.begin, r.size
+
+
+def plan_bands(n, p, mapping):
+    arr = (Range * p)()
+    ex = _c.c_int(0)
+    st = lib().uvt_plan_bands(n, p, MAPS.get(mapping, mapping), arr, _c.byref(ex))
+    if st != OK:
+        e = UvtError(st, "plan_bands")
+        e.ref_exit = ex.value
+        raise e
+    return [(r.begin, r.size) for r in arr]
+
+
+class Plan:
+    """Tile plan for C = tril(A) * B over p bands (host object)."""
+
+    def __init__(self, n, m, p=1, mapping="reg", shape="trapezoid"):
+        self.n, self.m, self.p = n, m, p
+        self.mapping, self.shape = mapping, shape
+        h = vp()
+        ex = _c.c_int(0)
+        st = lib().uvt_plan_create(n, m, p, MAPS.get(mapping, mapping), SHAPES.get(shape, shape),
+                                   _c.byref(h), _c.byref(ex))
+        if st != OK:
+            e = UvtError(st, "plan_create")
+            e.ref_exit = ex.value
+            raise e
+        self.ptr = h
+
+    def close(self):
+        if self.ptr:
+            lib().uvt_plan_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def phases(self):
+        return lib().uvt_plan_phases(self.ptr)
+
+    def tiles(self, phase):
+        cnt = lib().uvt_plan_num_tiles(self.ptr, phase)
+        arr = (Tile * max(cnt, 1))()
+        check(lib().uvt_plan_get_tiles(self.ptr, phase, arr, cnt), "get_tiles")
+        dt = np.dtype([(k, np.int32) for k in ("r0", "rows", "c0", "k_lo", "k_hi", "flags",
+                                               "band", "cfg")])
+        return np.frombuffer(bytes(arr), dtype=dt, count=cnt).copy()
+
+    def bands(self):
+        arr = (Range * self.p)()
+        check(lib().uvt_plan_get_bands(self.ptr, arr, self.p), "get_bands")
+        return [(r.begin, r.size) for r in arr]
+
+    def tile_shape(self):
+        a, b = i32(), i32()
+        check(lib().uvt_plan_tile_shape(self.ptr, _c.byref(a), _c.byref(b)))
+        return a.value, b.value
+
+    @property
+    def flops(self):
+        return lib().uvt_plan_flops(self.ptr)
+
+    @property
+    def issued_flops(self):
+        return lib().uvt_plan_issued_flops(self.ptr)
+
+
+# ------------------------------------------------------------------ device
+class DeviceBuffer:
+    def __init__(self, handle, nbytes):
+        self.h = handle
+        self.nbytes = nbytes
+        p = vp()
+        check(lib().uvt_malloc(handle.ptr, nbytes, _c.byref(p)), f"malloc {nbytes}")
+        self.ptr = p.value
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        check(lib().uvt_memcpy(self.h.ptr, self.ptr, arr.ctypes.data, arr.nbytes, 0), "h2d")
+
+    def download(self, shape, dtype=np.float64):
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        check(lib().uvt_memcpy(self.h.ptr, out.ctypes.data, self.ptr, out.nbytes, 1), "d2h")
+        return out
+
+    def zero(self):
+        check(lib().uvt_memset(self.h.ptr, self.ptr, 0, self.nbytes))
+
+    def free(self):
+        if self.ptr:
+            lib().uvt_free(self.h.ptr, self.ptr)
+            self.ptr = None
+
+
+class Handle:
+    def __init__(self, device=0):
+        h = vp()
+        check(lib().uvt_create(device, _c.byref(h)), "uvt_create")
+        self.ptr = h
+        s = vp()
+        check(lib().uvt_stream_create(self.ptr, _c.byref(s)), "stream")
+        self.stream = s
+
+    @property
+    def cus(self):
+        return lib().uvt_device_cus(self.ptr)
+
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def array(self, arr):
+        b = DeviceBuffer(self, max(np.asarray(arr).nbytes, 8))
+        b.upload(arr)
+        return b
+
+    def sync(self):
+        check(lib().uvt_stream_sync(self.ptr, self.stream), "sync")
+
+    def trmm_f64(self, plan, A, lda, B, ldb, C, ldc):
+        check(lib().uvt_trmm_f64(self.ptr, plan.ptr, A.ptr, lda, B.ptr, ldb, C.ptr, ldc,
+                                 self.stream), "trmm_f64")
+
+    def split_workspace(self, n, m, slices):
+        b = sz()
+        check(lib().uvt_trmm_split_workspace(n, m, slices, _c.byref(b)), "split_workspace")
+        return b.value
+
+    def trmm_split(self, n, m, slices, A, lda, B, ldb, C, ldc, work):
+        check(lib().uvt_trmm_split(self.ptr, n, m, slices, A.ptr, lda, B.ptr, ldb, C.ptr, ldc,
+                                   work.ptr, work.nbytes, self.stream), "trmm_split")
+
+    def inf_norm(self, n, m, C, ldc):
+        out = f64()
+        check(lib().uvt_inf_norm(self.ptr, n, m, C.ptr, ldc, _c.byref(out), self.stream), "norm")
+        return out.value
+
+    def row_abs_sums(self, n, m, C, ldc, out):
+        check(lib().uvt_row_abs_sums(self.ptr, n, m, C.ptr, ldc, out.ptr, self.stream))
+
+    def fill_closed_form(self, n, m, A, lda, B, ldb):
+        check(lib().uvt_fill_closed_form(self.ptr, n, m, A.ptr if A else None, lda,
+                                         B.ptr if B else None, ldb, self.stream), "fill_closed")
+
+    def fill_seeded(self, n, m, seed, spread, A, lda, B, ldb):
+        check(lib().uvt_fill_seeded(self.ptr, n, m, seed, spread, A.ptr if A else None, lda,
+                                    B.ptr if B else None, ldb, self.stream), "fill_seeded")
+
+    def event(self):
+        e = vp()
+        check(lib().uvt_event_create(self.ptr, _c.byref(e)))
+        return e
+
+    def record(self, ev):
+        check(lib().uvt_event_record(self.ptr, ev, self.stream))
+
+    def elapsed_ms(self, e0, e1):
+        ms = _c.c_float()
+        check(lib().uvt_event_elapsed(self.ptr, e0, e1, _c.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self.ptr:
+            lib().uvt_stream_destroy(self.ptr, self.stream)
+            lib().uvt_destroy(self.ptr)
+            self.ptr = None
+
+
+def gpu_available():
+    """True when libuvt can open device 0 (never raises)."""
+    try:
+        h = vp()
+        st = lib().uvt_create(0, _c.byref(h))
+        if st == OK:
+            lib().uvt_destroy(h)
+            return True
+    except Exception:
+        pass
+    return False

@@ -1,0 +1,189 @@
+// uvt_f64_tile.hpp -- FP64 MFMA tile of C = tril(A) * B for gfx950.
+//
+// One workgroup (4 wave64 waves) computes a TM x TN tile of C over a k range
+// [k_lo, k_hi), optionally masking A above the diagonal (A[i][k] = 0 for
+// k > i) and optionally accumulating into C.  This one routine covers every
+// product the reference issues per band (UVaTRMM/src/mm_triangular.c:347-393):
+//   * the GEMM box  C[b:b+h] = A[b:b+h, 0:b+1] * B[0:b+1]   (no mask, store)
+//   * the triangle  C[b+1:b+h] += tril(A_tri) * B_tri       (mask, accumulate)
+//   * the trapezoid C[r0:r0+TM] = A[r0:, 0:r0+TM] * B        (mask, store)
+//
+// Data path: A and B k-panels (BK = 16 deep) stream from HBM with 16-byte
+// loads into registers one panel ahead, are written to a double-buffered LDS
+// tile, and each wave multiplies its (TM/2) x (TN/2) sub-tile with
+// v_mfma_f64_16x16x4_f64 (one double of A and B per lane, four doubles of
+// C per lane).  A is stored k-major in LDS (transposed on the LDS write) so
+// both operand fragments are 16 consecutive doubles per k row.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace uvt {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+constexpr int kTileLower = 1;  // UVT_TILE_LOWER
+constexpr int kTileAccum = 2;  // UVT_TILE_ACCUM
+
+template <int TM, int TN>
+struct F64Cfg {
+    static constexpr int NT = 256;  // 4 waves: 2 x 2 over the tile
+    static constexpr int BK = 16;   // k depth of one LDS panel (4 MFMA steps)
+    static constexpr int WM = TM / 2, WN = TN / 2;
+    static constexpr int FM = WM / 16, FN = WN / 16;
+    // LDS row strides (doubles).  +16 doubles (128 B) puts the four k rows a
+    // fragment read touches on alternating halves of the 64 LDS banks.
+    static constexpr int LDA_S = TM + 16;
+    static constexpr int LDB_S = TN + 16;
+    static constexpr int A_ELEMS = BK * LDA_S;
+    static constexpr int B_ELEMS = BK * LDB_S;
+    static constexpr int SMEM_DOUBLES = 2 * (A_ELEMS + B_ELEMS);
+    static constexpr int A_VEC = TM * BK / 2 / NT;  // 2-double loads per thread
+    static constexpr int B_VEC = TN * BK / 2 / NT;
+    static_assert(A_VEC >= 1 && B_VEC >= 1, "tile too small for 256 threads");
+    static_assert(FM >= 1 && FN >= 1, "wave tile below 16x16");
+};
+
+// VEC: A, B 16-byte aligned and lda, ldb even, so 2-double loads are legal.
+template <int TM, int TN, bool VEC>
+__device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int k_hi, int flags,
+                                         const double* __restrict__ A, int64_t lda,
+                                         const double* __restrict__ B, int64_t ldb,
+                                         double* __restrict__ C, int64_t ldc, int64_t m,
+                                         double* smem)
+{
+    using G = F64Cfg<TM, TN>;
+    constexpr int BK = G::BK, NT = G::NT;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const bool lower = (flags & kTileLower) != 0;
+    double* As = smem;                    // [2][BK][LDA_S]
+    double* Bs = smem + 2 * G::A_ELEMS;   // [2][BK][LDB_S]
+
+    const int k_start = k_lo & ~1;  // even, so 2-double loads stay aligned
+    const int nk = (k_hi > k_start) ? (k_hi - k_start + BK - 1) / BK : 0;
+    const bool cols_full = (int64_t)c0 + TN <= m;
+
+    dbl4 acc[G::FM][G::FN];
+#pragma unroll
+    for (int i = 0; i < G::FM; ++i)
+#pragma unroll
+        for (int j = 0; j < G::FN; ++j) acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+
+    dbl2 ra[G::A_VEC], rb[G::B_VEC];
+
+    auto load = [&](int k0) {
+        const bool a_full = rows == TM && k0 >= k_lo && k0 + BK <= k_hi &&
+                            (!lower || k0 + BK - 1 <= r0);
+        const bool b_full = cols_full && k0 >= k_lo && k0 + BK <= k_hi;
+#pragma unroll
+        for (int q = 0; q < G::A_VEC; ++q) {
+            const int v = tid + q * NT;
+            const int row = v >> 3, kp = (v & 7) << 1;
+            const int gk = k0 + kp, gr = r0 + row;
+            const double* p = A + (int64_t)gr * lda + gk;
+            if (a_full) {
+                if (VEC) ra[q] = *reinterpret_cast<const dbl2*>(p);
+                else ra[q] = dbl2{p[0], p[1]};
+            } else {
+                const bool rv = row < rows;
+                const bool v0 = rv && gk >= k_lo && gk < k_hi && (!lower || gk <= gr);
+                const bool v1 = rv && gk + 1 >= k_lo && gk + 1 < k_hi && (!lower || gk + 1 <= gr);
+                ra[q] = dbl2{v0 ? p[0] : 0.0, v1 ? p[1] : 0.0};
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < G::B_VEC; ++q) {
+            const int v = tid + q * NT;
+            const int kr = v / (TN / 2), col = (v % (TN / 2)) * 2;
+            const int gk = k0 + kr;
+            const double* p = B + (int64_t)gk * ldb + (c0 + col);
+            if (b_full) {
+                if (VEC) rb[q] = *reinterpret_cast<const dbl2*>(p);
+                else rb[q] = dbl2{p[0], p[1]};
+            } else {
+                const bool kv = gk >= k_lo && gk < k_hi;
+                const bool v0 = kv && (int64_t)c0 + col < m;
+                const bool v1 = kv && (int64_t)c0 + col + 1 < m;
+                rb[q] = dbl2{v0 ? p[0] : 0.0, v1 ? p[1] : 0.0};
+            }
+        }
+    };
+
+    auto store = [&](int buf) {
+        double* as = As + buf * G::A_ELEMS;
+        double* bs = Bs + buf * G::B_ELEMS;
+#pragma unroll
+        for (int q = 0; q < G::A_VEC; ++q) {
+            const int v = tid + q * NT;
+            const int row = v >> 3, kp = (v & 7) << 1;
+            as[kp * G::LDA_S + row] = ra[q].x;
+            as[(kp + 1) * G::LDA_S + row] = ra[q].y;
+        }
+#pragma unroll
+        for (int q = 0; q < G::B_VEC; ++q) {
+            const int v = tid + q * NT;
+            const int kr = v / (TN / 2), col = (v % (TN / 2)) * 2;
+            *reinterpret_cast<dbl2*>(&bs[kr * G::LDB_S + col]) = rb[q];
+        }
+    };
+
+    auto compute = [&](int buf) {
+        const double* as = As + buf * G::A_ELEMS + wm * G::WM + (lane & 15);
+        const double* bs = Bs + buf * G::B_ELEMS + wn * G::WN + (lane & 15);
+#pragma unroll
+        for (int ks = 0; ks < BK / 4; ++ks) {
+            const int kk = ks * 4 + (lane >> 4);
+            double a[G::FM], b[G::FN];
+#pragma unroll
+            for (int i = 0; i < G::FM; ++i) a[i] = as[kk * G::LDA_S + i * 16];
+#pragma unroll
+            for (int j = 0; j < G::FN; ++j) b[j] = bs[kk * G::LDB_S + j * 16];
+#pragma unroll
+            for (int i = 0; i < G::FM; ++i)
+#pragma unroll
+                for (int j = 0; j < G::FN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+    };
+
+    if (nk > 0) {
+        load(k_start);
+        store(0);
+        __syncthreads();
+        for (int kt = 0; kt < nk; ++kt) {
+            const int cur = kt & 1;
+            const bool more = kt + 1 < nk;
+            if (more) load(k_start + (kt + 1) * BK);
+            compute(cur);
+            if (more) store(cur ^ 1);
+            __syncthreads();
+        }
+    }
+
+    // D layout of v_mfma_f64_16x16x4_f64: lane l, element v holds
+    // row 4*(l/16) + v, column l%16 of the 16x16 block.
+    const bool accum = (flags & kTileAccum) != 0;
+#pragma unroll
+    for (int i = 0; i < G::FM; ++i) {
+#pragma unroll
+        for (int j = 0; j < G::FN; ++j) {
+            const int col = wn * G::WN + j * 16 + (lane & 15);
+            const bool cv = (int64_t)c0 + col < m;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = wm * G::WM + i * 16 + (lane >> 4) * 4 + v;
+                if (cv && row < rows) {
+                    double* p = C + (int64_t)(r0 + row) * ldc + (c0 + col);
+                    if (accum) *p += acc[i][j][v];
+                    else *p = acc[i][j][v];
+                }
+            }
+        }
+    }
+}
+
+}  // namespace uvt

@@ -1,0 +1,45 @@
+/*
+ * uvt_internal.h -- definitions shared by the host planner (C) and the HIP
+ * side of libuvt.  Not installed; callers use include/uvt.h.
+ */
+#ifndef UVT_INTERNAL_H
+#define UVT_INTERNAL_H
+
+#include "uvt.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* FP64 tile kernel configurations (rows x columns of one workgroup tile). */
+enum { UVT_CFG_128x128 = 0, UVT_CFG_64x64 = 1, UVT_NUM_CFGS = 2 };
+int32_t uvt_cfg_tile_m(int32_t cfg);
+int32_t uvt_cfg_tile_n(int32_t cfg);
+
+struct uvt_plan_s {
+    int64_t n, m;
+    uint32_t p;
+    uvt_map map;
+    uvt_shape shape;
+    int32_t cfg;
+    uvt_range *bands;
+    int32_t phases;
+    int64_t ntiles[2];
+    uvt_tile *tiles[2];
+    double flops, issued;
+    /* device copies, owned by the plan, created by the HIP side */
+    int dev_id;
+    void *dev_tiles[2];
+};
+
+/* Planner entry used by the batch path: plan one problem with a given tile
+ * configuration into caller-owned arrays (NULL out = count only). */
+int64_t uvt_plan_trapezoid_tiles(int64_t n, int64_t m, int32_t cfg, uvt_tile *out);
+
+/* Releases device copies of a plan (defined on the HIP side). */
+void uvt_plan_release_device(uvt_plan plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
