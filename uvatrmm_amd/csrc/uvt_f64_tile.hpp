@@ -164,8 +164,8 @@ __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int
         }
     }
 
-    // D layout of v_mfma_f64_16x16x4_f64: lane l, element v holds
-    // row 4*(l/16) + v, column l%16 of the 16x16 block.
+    // D layout of v_mfma_f64_16x16x4_f64 (measured on gfx950): lane l,
+    // element v holds row l/16 + 4*v, column l%16 of the 16x16 block.
     const bool accum = (flags & kTileAccum) != 0;
 #pragma unroll
     for (int i = 0; i < G::FM; ++i) {
@@ -175,7 +175,7 @@ __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int
             const bool cv = (int64_t)c0 + col < m;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int row = wm * G::WM + i * 16 + (lane >> 4) * 4 + v;
+                const int row = wm * G::WM + i * 16 + (lane >> 4) + 4 * v;
                 if (cv && row < rows) {
                     double* p = C + (int64_t)(r0 + row) * ldc + (c0 + col);
                     if (accum) *p += acc[i][j][v];
