@@ -97,7 +97,7 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--spread", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-cols-per-rank", type=int, default=96)
+    ap.add_argument("--cpu-cols-per-rank", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the TRMM path shards by columns of B with no "
@@ -152,7 +152,7 @@ def main():
         except Exception as e:  # the baseline is reported, never the measured path
             log("cpu baseline failed:", repr(e))
 
-    kernel = "k_trmm_tiles_f64<128,128,VEC>"
+    kernel = "k_trmm_tiles_f64<128,128,true> (8 waves of 32x64)"
     out = {
         "metric": "TRMM TFLOP/s at N=32768 (plain FP64, % of FP64 MFMA peak)",
         "value": round(value, 4),

@@ -15,17 +15,25 @@ using namespace uvt;
 // Kernels
 // ---------------------------------------------------------------------
 
+// Workgroup shapes of the FP64 tile kernel: a 128x128 tile runs as 8 waves
+// of 32x64 (4 waves per SIMD with two workgroups per CU, which hides the
+// global-load latency better than 4 waves of 64x64); the 64x64 tile for
+// small problems runs as 4 waves of 32x32.
+template <int TM> struct TileWaves;
+template <> struct TileWaves<128> { static constexpr int M = 4, N = 2, BLOCKS_PER_CU = 2; };
+template <> struct TileWaves<64> { static constexpr int M = 2, N = 2, BLOCKS_PER_CU = 4; };
+
 template <int TM, int TN, bool VEC>
-__global__ __launch_bounds__(256, 2) void k_trmm_tiles_f64(const uvt_tile* __restrict__ tiles,
-                                                           const double* __restrict__ A, int64_t lda,
-                                                           const double* __restrict__ B, int64_t ldb,
-                                                           double* __restrict__ C, int64_t ldc,
-                                                           int64_t m)
+__global__ __launch_bounds__(64 * TileWaves<TM>::M * TileWaves<TM>::N, TileWaves<TM>::BLOCKS_PER_CU)
+void k_trmm_tiles_f64(const uvt_tile* __restrict__ tiles, const double* __restrict__ A, int64_t lda,
+                      const double* __restrict__ B, int64_t ldb, double* __restrict__ C,
+                      int64_t ldc, int64_t m)
 {
-    __shared__ double smem[F64Cfg<TM, TN>::SMEM_DOUBLES];
+    using W = TileWaves<TM>;
+    __shared__ double smem[F64Cfg<TM, TN, W::M, W::N>::SMEM_DOUBLES];
     const uvt_tile t = tiles[blockIdx.x];
-    f64_tile<TM, TN, VEC>(t.r0, t.rows, t.c0, t.k_lo, t.k_hi, t.flags, A, lda, B, ldb, C, ldc, m,
-                          smem);
+    f64_tile<TM, TN, VEC, W::M, W::N>(t.r0, t.rows, t.c0, t.k_lo, t.k_hi, t.flags, A, lda, B, ldb,
+                                      C, ldc, m, smem);
 }
 
 // One wave per row: sum_j |C[i][j]|, lanes strided over columns, then a
@@ -214,7 +222,7 @@ static void launch_tiles(const uvt_tile* dtiles, int64_t ntiles, bool vec, const
                          int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc,
                          int64_t m, hipStream_t s)
 {
-    dim3 grid((unsigned)ntiles), block(256);
+    dim3 grid((unsigned)ntiles), block(64 * TileWaves<TM>::M * TileWaves<TM>::N);
     if (vec)
         hipLaunchKernelGGL((k_trmm_tiles_f64<TM, TN, true>), grid, block, 0, s, dtiles, A, lda, B,
                            ldb, C, ldc, m);

@@ -1,6 +1,6 @@
 // uvt_f64_tile.hpp -- FP64 MFMA tile of C = tril(A) * B for gfx950.
 //
-// One workgroup (4 wave64 waves) computes a TM x TN tile of C over a k range
+// One workgroup (WAVES_M x WAVES_N wave64 waves) computes a TM x TN tile of C over a k range
 // [k_lo, k_hi), optionally masking A above the diagonal (A[i][k] = 0 for
 // k > i) and optionally accumulating into C.  This one routine covers every
 // product the reference issues per band (UVaTRMM/src/mm_triangular.c:347-393):
@@ -10,7 +10,7 @@
 //
 // Data path: A and B k-panels (BK = 16 deep) stream from HBM with 16-byte
 // loads into registers one panel ahead, are written to a double-buffered LDS
-// tile, and each wave multiplies its (TM/2) x (TN/2) sub-tile with
+// tile, and each wave multiplies its (TM/WAVES_M) x (TN/WAVES_N) sub-tile with
 // v_mfma_f64_16x16x4_f64 (one double of A and B per lane, four doubles of
 // C per lane).  A is stored k-major in LDS (transposed on the LDS write) so
 // both operand fragments are 16 consecutive doubles per k row.
@@ -27,11 +27,11 @@ typedef double dbl2 __attribute__((ext_vector_type(2)));
 constexpr int kTileLower = 1;  // UVT_TILE_LOWER
 constexpr int kTileAccum = 2;  // UVT_TILE_ACCUM
 
-template <int TM, int TN>
+template <int TM, int TN, int WAVES_M = 2, int WAVES_N = 2, int BK_ = 16>
 struct F64Cfg {
-    static constexpr int NT = 256;  // 4 waves: 2 x 2 over the tile
-    static constexpr int BK = 16;   // k depth of one LDS panel (4 MFMA steps)
-    static constexpr int WM = TM / 2, WN = TN / 2;
+    static constexpr int NT = 64 * WAVES_M * WAVES_N;
+    static constexpr int BK = BK_;  // k depth of one LDS panel (BK/4 MFMA steps)
+    static constexpr int WM = TM / WAVES_M, WN = TN / WAVES_N;
     static constexpr int FM = WM / 16, FN = WN / 16;
     // LDS row strides (doubles).  +16 doubles (128 B) puts the four k rows a
     // fragment read touches on alternating halves of the 64 LDS banks.
@@ -42,23 +42,24 @@ struct F64Cfg {
     static constexpr int SMEM_DOUBLES = 2 * (A_ELEMS + B_ELEMS);
     static constexpr int A_VEC = TM * BK / 2 / NT;  // 2-double loads per thread
     static constexpr int B_VEC = TN * BK / 2 / NT;
-    static_assert(A_VEC >= 1 && B_VEC >= 1, "tile too small for 256 threads");
+    static_assert(A_VEC >= 1 && B_VEC >= 1, "tile too small for the thread count");
+    static_assert(TM * BK / 2 % NT == 0 && TN * BK / 2 % NT == 0, "uneven panel split");
     static_assert(FM >= 1 && FN >= 1, "wave tile below 16x16");
 };
 
 // VEC: A, B 16-byte aligned and lda, ldb even, so 2-double loads are legal.
-template <int TM, int TN, bool VEC>
+template <int TM, int TN, bool VEC, int WAVES_M = 2, int WAVES_N = 2, int BK_ = 16>
 __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int k_hi, int flags,
                                          const double* __restrict__ A, int64_t lda,
                                          const double* __restrict__ B, int64_t ldb,
                                          double* __restrict__ C, int64_t ldc, int64_t m,
                                          double* smem)
 {
-    using G = F64Cfg<TM, TN>;
+    using G = F64Cfg<TM, TN, WAVES_M, WAVES_N, BK_>;
     constexpr int BK = G::BK, NT = G::NT;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
+    const int wm = wave / WAVES_N, wn = wave % WAVES_N;
     const bool lower = (flags & kTileLower) != 0;
     double* As = smem;                    // [2][BK][LDA_S]
     double* Bs = smem + 2 * G::A_ELEMS;   // [2][BK][LDB_S]
@@ -82,7 +83,7 @@ __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int
 #pragma unroll
         for (int q = 0; q < G::A_VEC; ++q) {
             const int v = tid + q * NT;
-            const int row = v >> 3, kp = (v & 7) << 1;
+            const int row = v / (BK / 2), kp = (v % (BK / 2)) * 2;
             const int gk = k0 + kp, gr = r0 + row;
             const double* p = A + (int64_t)gr * lda + gk;
             if (a_full) {
@@ -119,7 +120,7 @@ __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int
 #pragma unroll
         for (int q = 0; q < G::A_VEC; ++q) {
             const int v = tid + q * NT;
-            const int row = v >> 3, kp = (v & 7) << 1;
+            const int row = v / (BK / 2), kp = (v % (BK / 2)) * 2;
             as[kp * G::LDA_S + row] = ra[q].x;
             as[(kp + 1) * G::LDA_S + row] = ra[q].y;
         }
