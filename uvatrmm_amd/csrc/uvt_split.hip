@@ -49,7 +49,9 @@ struct Mods {
     int m[kMaxMods];
     float invf[kMaxMods];
     double invd[kMaxMods];
-    uint8_t garner[kMaxMods][kMaxMods];  // garner[p][l] = m_p^-1 mod m_l (p < l)
+    uint8_t prodmod[kMaxMods][kMaxMods];  // (m_0 ... m_{p-1}) mod m_l, p < l
+    uint8_t qinv[kMaxMods];               // (m_0 ... m_{l-1})^-1 mod m_l
+    int lift[kMaxMods];                   // multiple of m_l >= 2^20 (keeps sums >= 0)
     uint64_t half_hi, half_lo;           // M/2 (floor) as a 128-bit integer
     uint64_t M_hi, M_lo;
 };
@@ -189,43 +191,27 @@ __global__ __launch_bounds__(256) void k_split_resid_b(int64_t kmax, int64_t m,
 
 // ---- int8 modular products ---------------------------------------------------
 
-// One workgroup: a 256 x 256 tile of (A' B') mod m_l over k in [0, r0), with
-// 8 waves of 64 x 128 (2 x 4 blocks of v_mfma_i32_32x32x32_i8).
-__global__ __launch_bounds__(512, 1) void k_split_gemm_i8(
-    const int8_t* __restrict__ Ar, int64_t lda_r, int64_t a_mod_stride,
-    const int8_t* __restrict__ Br, int64_t ldb_r, int64_t b_mod_stride,
-    uint8_t* __restrict__ R, int64_t ldr, int64_t r_mod_stride, int64_t n, int64_t m, int nrt,
-    int nct, Mods mods)
+// The k loop of one 256 x 256 tile: A' and B' residue panels (128 bytes of k
+// per stage) through a double-buffered LDS tile; 8 waves of 64 x 128, each a
+// 2 x 4 block of v_mfma_i32_32x32x32_i8.  FULL: the tile lies inside C, so the
+// loads need no bounds checks.
+template <bool FULL>
+__device__ __forceinline__ void i8_tile_loop(const int8_t* __restrict__ A, int64_t lda_r,
+                                             const int8_t* __restrict__ B, int64_t ldb_r, int64_t rows_left,
+                                             int64_t cols_left, int K, int8_t* smem, v16i (&acc)[2][4])
 {
-    __shared__ __attribute__((aligned(16))) int8_t smem[2 * kI8Stage];
-    const int per_rt = nct * mods.n;
-    const int rt = nrt - 1 - (int)(blockIdx.x / per_rt);  // heavy (long k) tiles first
-    const int rem = (int)(blockIdx.x % per_rt);
-    const int l = rem / nct, ct = rem % nct;
-    const int64_t r0 = (int64_t)rt * kTileI, c0 = (int64_t)ct * kTileI;
-    const int K = (int)r0;
-    const int8_t* A = Ar + l * a_mod_stride + r0 * lda_r;
-    const int8_t* B = Br + l * b_mod_stride + c0 * ldb_r;
-
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-
-    v16i acc[2][4];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = v16i{0};
-
     v4i ra[4], rb[4];
     auto load = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int c = tid + q * 512;
             const int row = c >> 3, seg = (c & 7) * 16;
-            ra[q] = (r0 + row < n) ? *reinterpret_cast<const v4i*>(A + row * lda_r + k0 + seg)
-                                   : v4i{0};
-            rb[q] = (c0 + row < m) ? *reinterpret_cast<const v4i*>(B + row * ldb_r + k0 + seg)
-                                   : v4i{0};
+            if (FULL || row < rows_left) ra[q] = *reinterpret_cast<const v4i*>(A + row * lda_r + k0 + seg);
+            else ra[q] = v4i{0};
+            if (FULL || row < cols_left) rb[q] = *reinterpret_cast<const v4i*>(B + row * ldb_r + k0 + seg);
+            else rb[q] = v4i{0};
         }
     };
     auto store = [&](int buf) {
@@ -257,7 +243,6 @@ __global__ __launch_bounds__(512, 1) void k_split_gemm_i8(
                     acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j], acc[i][j], 0, 0, 0);
         }
     };
-
     const int nk = K / kI8BK;  // K is a multiple of 256
     load(0);
     store(0);
@@ -269,9 +254,46 @@ __global__ __launch_bounds__(512, 1) void k_split_gemm_i8(
         if (more) store((kt + 1) & 1);
         __syncthreads();
     }
+}
+
+// One workgroup: a 256 x 256 tile of (A' B') mod m_l over k in [0, r0).
+__global__ __launch_bounds__(512, 1) void k_split_gemm_i8(
+    const int8_t* __restrict__ Ar, int64_t lda_r, int64_t a_mod_stride,
+    const int8_t* __restrict__ Br, int64_t ldb_r, int64_t b_mod_stride,
+    uint8_t* __restrict__ R, int64_t ldr, int64_t r_mod_stride, int64_t n, int64_t m, int nrt,
+    int nct, Mods mods)
+{
+    __shared__ __attribute__((aligned(16))) int8_t smem[2 * kI8Stage];
+    // XCD-aware mapping.  Workgroups go to the 8 XCDs round-robin, so XCD x
+    // runs blocks x, x+8, ...; its s-th block takes tile s%32 of tile group
+    // (s/32)*8 + x.  A group is 4 row tiles x 8 column tiles of one modulus,
+    // i.e. the 32 workgroups one XCD holds at once, so they share 4 A panels
+    // and 8 B panels through that XCD's L2.  Groups run longest-k first.
+    const int xcd = (int)(blockIdx.x & 7), slot = (int)(blockIdx.x >> 3);
+    const int g = (slot >> 5) * 8 + xcd, t = slot & 31;
+    const int ngc = (nct + 7) / 8, per_rg = ngc * mods.n;
+    const int rg = g / per_rg, grem = g % per_rg;
+    const int l = grem / ngc, ct = (grem % ngc) * 8 + (t & 7);
+    const int rt = nrt - 1 - (rg * 4 + (t >> 3));
+    if (rt < 1 || ct >= nct) return;  // padding of the last groups (uniform per block)
+    const int64_t r0 = (int64_t)rt * kTileI, c0 = (int64_t)ct * kTileI;
+    const int8_t* A = Ar + l * a_mod_stride + r0 * lda_r;
+    const int8_t* B = Br + l * b_mod_stride + c0 * ldb_r;
+
+    v16i acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = v16i{0};
+    if (r0 + kTileI <= n && c0 + kTileI <= m)
+        i8_tile_loop<true>(A, lda_r, B, ldb_r, n - r0, m - c0, (int)r0, smem, acc);
+    else
+        i8_tile_loop<false>(A, lda_r, B, ldb_r, n - r0, m - c0, (int)r0, smem, acc);
 
     // D layout of v_mfma_i32_32x32x32_i8 (measured on gfx950): lane l, element
     // v holds row 8*(v/4) + 4*(l/32) + v%4, column l%32.
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
     const int mod = mods.m[l];
     const float invf = mods.invf[l];
     uint8_t* out = R + l * r_mod_stride;
@@ -336,10 +358,17 @@ __global__ __launch_bounds__(256) void k_split_reconstruct(int64_t n, int64_t m,
 {
     // rows of the first 256-row tile have no box part
     for (int64_t i = kTileI + blockIdx.y; i < n; i += gridDim.y)
-    for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < m; j += (int64_t)gridDim.x * 256) {
+    for (int64_t j4 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 4; j4 < m;
+         j4 += (int64_t)gridDim.x * 256 * 4)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int64_t j = j4 + e;
+        if (j >= m) break;
         const uint8_t* r = R + i * ldr + j;
-        // Garner: mixed-radix digits v_l of the residues (fully unrolled so
-        // every digit lives in a named register).
+        // Mixed-radix digits of x (x = sum_l v_l m_0...m_{l-1}):
+        //   v_l = (r_l - sum_{p<l} v_p (m_0...m_{p-1} mod m_l)) (m_0...m_{l-1})^-1 mod m_l,
+        // a multiply-accumulate chain (< 2^20) and two reductions per digit,
+        // fully unrolled so every digit stays in a register.
         int v[kMaxMods];
 #pragma unroll
         for (int l = 0; l < kMaxMods; ++l) {
@@ -347,12 +376,10 @@ __global__ __launch_bounds__(256) void k_split_reconstruct(int64_t n, int64_t m,
             if (l < mods.n) {
                 const int ml = mods.m[l];
                 const float fl = mods.invf[l];
-                t = r[l * r_mod_stride];
+                int acc = mods.lift[l] + (int)r[l * r_mod_stride];
 #pragma unroll
-                for (int p = 0; p < l; ++p) {
-                    const int d = t - v[p] + 2 * ml;  // >= 0 (v[p] < 256 <= 2*ml), same residue
-                    t = mod_small(mod_small(d, ml, fl) * mods.garner[p][l], ml, fl);
-                }
+                for (int p = 0; p < l; ++p) acc -= v[p] * (int)mods.prodmod[p][l];
+                t = mod_small(mod_small(acc, ml, fl) * (int)mods.qinv[l], ml, fl);
             }
             v[l] = t;
         }
@@ -433,8 +460,15 @@ static Mods make_mods(int nmod, double* log2M)
         M *= (unsigned)kModuli[l];
         lg += log2((double)kModuli[l]);
     }
-    for (int p = 0; p < nmod; ++p)
-        for (int l = p + 1; l < nmod; ++l) md.garner[p][l] = (uint8_t)inv_mod(md.m[p], md.m[l]);
+    for (int l = 0; l < nmod; ++l) {
+        int w = 1;  // (m_0 ... m_{p-1}) mod m_l
+        for (int p = 0; p < l; ++p) {
+            md.prodmod[p][l] = (uint8_t)w;
+            w = (w * md.m[p]) % md.m[l];
+        }
+        md.qinv[l] = (uint8_t)inv_mod(w, md.m[l]);
+        md.lift[l] = ((1 << 20) / md.m[l] + 1) * md.m[l];
+    }
     const unsigned __int128 half = M / 2;
     md.half_hi = (uint64_t)(half >> 64);
     md.half_lo = (uint64_t)half;
@@ -509,14 +543,15 @@ extern "C" uvt_status uvt_trmm_split(uvt_handle h, int64_t n, int64_t m, int32_t
                        (int64_t)m * L.ldb_r, mods);
     UVT_HIP(hipGetLastError());
     const int nrt = (int)((n + kTileI - 1) / kTileI), nct = (int)((m + kTileI - 1) / kTileI);
-    const int64_t blocks = (int64_t)(nrt - 1) * nct * slices;
+    const int64_t groups = (int64_t)((nrt - 1 + 3) / 4) * ((nct + 7) / 8) * slices;
+    const int64_t blocks = (groups + 7) / 8 * 8 * 32;
     if (blocks > 0x7FFFFFFF) return UVT_ERR_SIZE;
     hipLaunchKernelGGL(k_split_gemm_i8, dim3((unsigned)blocks), dim3(512), 0, s, Ar, L.lda_r,
                        (int64_t)n * L.lda_r, Br, L.ldb_r, (int64_t)m * L.ldb_r, R, m, n * m, n, m, nrt,
                        nct, mods);
     UVT_HIP(hipGetLastError());
     {
-        int64_t gx = (m + 255) / 256;
+        int64_t gx = (m + 1023) / 1024;
         if (gx > 32) gx = 32;
         const int64_t gy = n - kTileI < 32768 ? n - kTileI : 32768;
         hipLaunchKernelGGL(k_split_reconstruct, dim3((unsigned)gx, (unsigned)gy), dim3(256),
