@@ -35,6 +35,7 @@ using namespace uvt;
 namespace {
 
 constexpr int kMaxMods = 16;
+constexpr int kMaxSlices = 14;  // the two 64-bit halves of the reconstruction stay below 2^56
 constexpr int kTileI = 256;  // int8 tile rows = row-tile granularity of the box/diagonal split
 constexpr int kI8BK = 128;   // k bytes per LDS stage
 constexpr int kI8S = kI8BK + 16;  // LDS row stride (bytes): conflict-free 16-byte fragment reads
@@ -51,9 +52,12 @@ struct Mods {
     double invd[kMaxMods];
     uint8_t prodmod[kMaxMods][kMaxMods];  // (m_0 ... m_{p-1}) mod m_l, p < l
     uint8_t qinv[kMaxMods];               // (m_0 ... m_{l-1})^-1 mod m_l
-    int lift[kMaxMods];                   // multiple of m_l >= 2^20 (keeps sums >= 0)
-    uint64_t half_hi, half_lo;           // M/2 (floor) as a 128-bit integer
-    uint64_t M_hi, M_lo;
+    uint32_t lift[kMaxMods];              // multiple of m_l >= 2^20 (keeps sums >= 0)
+    uint32_t magic[kMaxMods];             // ceil(2^39 / m_l): x mod m_l for x < 2^22
+    int split;                            // digits [0, split) form the low word
+    uint64_t wlo;                         // m_0 ... m_{split-1}   (< 2^56)
+    uint64_t qhi;                         // m_split ... m_{n-1}  (< 2^56)
+    double wlo_hi, wlo_lo;                // wlo as a double-word
 };
 
 typedef int v4i __attribute__((ext_vector_type(4)));
@@ -340,14 +344,21 @@ __global__ __launch_bounds__(512, 2) void k_split_diag_f64(int64_t n, int64_t m,
 
 // ---- reconstruction -----------------------------------------------------------
 
-__device__ __forceinline__ int mod_small(int x, int m, float invf)
+// x mod m for 0 <= x < 2^22: with magic = ceil(2^39/m), floor(x/m) equals
+// (x * magic) >> 39 exactly on that range (the rounding up of magic adds
+// less than 2^-17 < 1/m to x/m).
+__device__ __forceinline__ uint32_t mod_magic(uint32_t x, uint32_t m, uint32_t magic)
 {
-    int r = x - m * (int)floorf((float)x * invf);
-    if (r < 0) r += m;
-    if (r >= m) r -= m;
-    return r;
+    return x - (__umulhi(x, magic) >> 7) * m;
 }
 
+// C[i][j] += C'_ij * 2^-(alpha_i + beta_j) for rows past the first 256-row
+// tile.  Per element: mixed-radix digits v_l of C' from its residues
+//   v_l = (r_l - sum_{p<l} v_p (m_0...m_{p-1} mod m_l)) (m_0...m_{l-1})^-1 mod m_l,
+// then x = X_lo + W X_hi with X_lo = digits [0, split) and X_hi = the rest,
+// both exact in 64 bits, and the signed value (x, or x - M when x > M/2)
+// rounded once to FP64 with a double-word W.  Four columns per thread.
+template <bool VEC4>
 __global__ __launch_bounds__(256) void k_split_reconstruct(int64_t n, int64_t m,
                                                            const uint8_t* __restrict__ R, int64_t ldr,
                                                            int64_t r_mod_stride,
@@ -356,58 +367,63 @@ __global__ __launch_bounds__(256) void k_split_reconstruct(int64_t n, int64_t m,
                                                            double* __restrict__ C, int64_t ldc,
                                                            Mods mods)
 {
-    // rows of the first 256-row tile have no box part
     for (int64_t i = kTileI + blockIdx.y; i < n; i += gridDim.y)
     for (int64_t j4 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 4; j4 < m;
-         j4 += (int64_t)gridDim.x * 256 * 4)
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const int64_t j = j4 + e;
-        if (j >= m) break;
-        const uint8_t* r = R + i * ldr + j;
-        // Mixed-radix digits of x (x = sum_l v_l m_0...m_{l-1}):
-        //   v_l = (r_l - sum_{p<l} v_p (m_0...m_{p-1} mod m_l)) (m_0...m_{l-1})^-1 mod m_l,
-        // a multiply-accumulate chain (< 2^20) and two reductions per digit,
-        // fully unrolled so every digit stays in a register.
-        int v[kMaxMods];
+         j4 += (int64_t)gridDim.x * 256 * 4) {
+        const int ne = m - j4 < 4 ? (int)(m - j4) : 4;
+        uint32_t res[kMaxMods][4];
 #pragma unroll
         for (int l = 0; l < kMaxMods; ++l) {
-            int t = 0;
             if (l < mods.n) {
-                const int ml = mods.m[l];
-                const float fl = mods.invf[l];
-                int acc = mods.lift[l] + (int)r[l * r_mod_stride];
+                const uint8_t* r = R + l * r_mod_stride + i * ldr + j4;
+                if (VEC4 && ne == 4) {
+                    const uint32_t w = *reinterpret_cast<const uint32_t*>(r);
 #pragma unroll
-                for (int p = 0; p < l; ++p) acc -= v[p] * (int)mods.prodmod[p][l];
-                t = mod_small(mod_small(acc, ml, fl) * (int)mods.qinv[l], ml, fl);
-            }
-            v[l] = t;
-        }
-        // x = v_0 + m_0 (v_1 + m_1 (v_2 + ...)) in two 64-bit limbs.
-        uint64_t hi = 0, lo = 0;
+                    for (int e = 0; e < 4; ++e) res[l][e] = (w >> (8 * e)) & 255u;
+                } else {
 #pragma unroll
-        for (int l = kMaxMods - 1; l >= 0; --l) {
-            if (l < mods.n) {
-                const uint64_t ml = (uint64_t)mods.m[l];
-                const uint64_t carry = __umul64hi(lo, ml);
-                hi = hi * ml + carry;
-                lo = lo * ml;
-                const uint64_t nl = lo + (uint64_t)v[l];
-                hi += nl < lo ? 1u : 0u;
-                lo = nl;
+                    for (int e = 0; e < 4; ++e) res[l][e] = e < ne ? r[e] : 0u;
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) res[l][e] = 0u;
             }
         }
-        double d;
-        if (hi > mods.half_hi || (hi == mods.half_hi && lo > mods.half_lo)) {
-            // value = x - M = -(M - x)
-            const uint64_t yl = mods.M_lo - lo;
-            const uint64_t yh = mods.M_hi - hi - (mods.M_lo < lo ? 1u : 0u);
-            d = -((double)yh * 0x1p64 + (double)yl);
-        } else {
-            d = (double)hi * 0x1p64 + (double)lo;
+        const int ai = alpha[i];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            if (e >= ne) break;
+            uint32_t v[kMaxMods];
+#pragma unroll
+            for (int l = 0; l < kMaxMods; ++l) {
+                uint32_t t = 0;
+                if (l < mods.n) {
+                    uint32_t acc = mods.lift[l] + res[l][e];
+#pragma unroll
+                    for (int p = 0; p < l; ++p) acc -= v[p] * (uint32_t)mods.prodmod[p][l];
+                    t = mod_magic(acc, (uint32_t)mods.m[l], mods.magic[l]);
+                    t = mod_magic(t * (uint32_t)mods.qinv[l], (uint32_t)mods.m[l], mods.magic[l]);
+                }
+                v[l] = t;
+            }
+            uint64_t xlo = 0, xhi = 0;
+#pragma unroll
+            for (int l = kMaxMods - 1; l >= 0; --l) {
+                if (l < mods.split) xlo = xlo * (uint64_t)mods.m[l] + v[l];
+                else if (l < mods.n) xhi = xhi * (uint64_t)mods.m[l] + v[l];
+            }
+            double h, low;
+            if (2 * xhi > mods.qhi) {  // x > M/2: value x - M = (X_hi - Q + 1) W + (X_lo - W)
+                h = (double)((int64_t)xhi - (int64_t)mods.qhi + 1);
+                low = (double)((int64_t)xlo - (int64_t)mods.wlo);
+            } else {
+                h = (double)xhi;
+                low = (double)xlo;
+            }
+            const double d = fma(h, mods.wlo_hi, fma(h, mods.wlo_lo, low));
+            const int64_t j = j4 + e;
+            C[i * ldc + j] += ldexp(d, -(ai + beta[j]));
         }
-        double* c = C + i * ldc + j;
-        *c += ldexp(d, -(alpha[i] + beta[j]));
     }
 }
 
@@ -451,13 +467,12 @@ static Mods make_mods(int nmod, double* log2M)
     Mods md;
     memset(&md, 0, sizeof md);
     md.n = nmod;
-    unsigned __int128 M = 1;
     double lg = 0.0;
     for (int l = 0; l < nmod; ++l) {
         md.m[l] = kModuli[l];
         md.invf[l] = 1.0f / (float)kModuli[l];
         md.invd[l] = 1.0 / (double)kModuli[l];
-        M *= (unsigned)kModuli[l];
+        md.magic[l] = (uint32_t)(((1ull << 39) + kModuli[l] - 1) / kModuli[l]);
         lg += log2((double)kModuli[l]);
     }
     for (int l = 0; l < nmod; ++l) {
@@ -467,13 +482,15 @@ static Mods make_mods(int nmod, double* log2M)
             w = (w * md.m[p]) % md.m[l];
         }
         md.qinv[l] = (uint8_t)inv_mod(w, md.m[l]);
-        md.lift[l] = ((1 << 20) / md.m[l] + 1) * md.m[l];
+        md.lift[l] = (uint32_t)(((1 << 20) / md.m[l] + 1) * md.m[l]);
     }
-    const unsigned __int128 half = M / 2;
-    md.half_hi = (uint64_t)(half >> 64);
-    md.half_lo = (uint64_t)half;
-    md.M_hi = (uint64_t)(M >> 64);
-    md.M_lo = (uint64_t)M;
+    md.split = nmod - 1 < 7 ? nmod - 1 : 7;
+    md.wlo = 1;
+    for (int l = 0; l < md.split; ++l) md.wlo *= (uint64_t)md.m[l];
+    md.qhi = 1;
+    for (int l = md.split; l < nmod; ++l) md.qhi *= (uint64_t)md.m[l];
+    md.wlo_hi = (double)md.wlo;
+    md.wlo_lo = (double)((int64_t)md.wlo - (int64_t)(uint64_t)md.wlo_hi);
     *log2M = lg;
     return md;
 }
@@ -482,7 +499,7 @@ static Mods make_mods(int nmod, double* log2M)
 
 extern "C" uvt_status uvt_trmm_split_workspace(int64_t n, int64_t m, int32_t slices, size_t* bytes)
 {
-    if (!bytes || n <= 0 || m <= 0 || slices < 2 || slices > kMaxMods) return UVT_ERR_ARG;
+    if (!bytes || n <= 0 || m <= 0 || slices < 2 || slices > kMaxSlices) return UVT_ERR_ARG;
     if (n > 131072 || m > 0x3FFFFFFF) return UVT_ERR_SIZE;  // int32 accumulation bound on k
     *bytes = layout(n, m, slices).total;
     return UVT_OK;
@@ -554,8 +571,12 @@ extern "C" uvt_status uvt_trmm_split(uvt_handle h, int64_t n, int64_t m, int32_t
         int64_t gx = (m + 1023) / 1024;
         if (gx > 32) gx = 32;
         const int64_t gy = n - kTileI < 32768 ? n - kTileI : 32768;
-        hipLaunchKernelGGL(k_split_reconstruct, dim3((unsigned)gx, (unsigned)gy), dim3(256),
-                           0, s, n, m, R, m, n * m, alpha, beta, dC, ldc, mods);
+        if (m % 4 == 0)
+            hipLaunchKernelGGL(k_split_reconstruct<true>, dim3((unsigned)gx, (unsigned)gy), dim3(256),
+                               0, s, n, m, R, m, n * m, alpha, beta, dC, ldc, mods);
+        else
+            hipLaunchKernelGGL(k_split_reconstruct<false>, dim3((unsigned)gx, (unsigned)gy), dim3(256),
+                               0, s, n, m, R, m, n * m, alpha, beta, dC, ldc, mods);
     }
     UVT_HIP(hipGetLastError());
     return UVT_OK;
