@@ -121,6 +121,52 @@ def split_leg(args, h, uvt, dA, dB, dC, flops, plain_ms):
             "int8_peak_tops": INT8_PEAK_TOPS}
 
 
+def batch_leg(args, h, uvt):
+    """512 uneven TRMMs (orders log-uniform in [64, 4096], B columns uniform in
+    [16, 1024], alternating FP64 / split) in one hipGraph launch."""
+    rng = np.random.default_rng(args.seed + 17)
+    ns = np.rint(np.exp(rng.uniform(np.log(64), np.log(4096), args.batch))).astype(int)
+    ms = rng.integers(16, 1025, args.batch)
+    order = rng.permutation(args.batch)  # which half is split: a fixed seeded permutation
+    split_set = set(order[: args.batch // 2].tolist())
+    table = os.path.join(ROOT, "uvatrmm_amd", "tune", "f64_tiles.json")
+    table = table if os.path.exists(table) else None
+    bufs, probs = [], []
+    for k in range(args.batch):
+        n, m = int(ns[k]), int(ms[k])
+        dA, dB, dC = h.alloc(n * n * 8), h.alloc(n * m * 8), h.alloc(n * m * 8)
+        h.fill_seeded(n, m, args.seed + k, args.spread, dA, n, dB, m)
+        bufs += [dA, dB, dC]
+        probs.append(uvt.Problem(dA.ptr, n, dB.ptr, m, dC.ptr, m, n, m,
+                                 uvt.PREC_SPLIT if k in split_set else uvt.PREC_F64, 0))
+    h.sync()
+    t0 = time.perf_counter()
+    batch = uvt.Batch(h, probs, slices=args.split_moduli or 13, table=table)
+    build_s = time.perf_counter() - t0
+    info = batch.info()
+    for _ in range(max(1, args.warmup)):
+        batch.run()
+    h.sync()
+    evs = [(h.event(), h.event()) for _ in range(args.steps)]
+    for e0, e1 in evs:
+        h.record(e0)
+        batch.run()
+        h.record(e1)
+    h.sync()
+    ms_step = float(np.mean([h.elapsed_ms(e0, e1) for e0, e1 in evs]))
+    batch.close()
+    for b in bufs:
+        b.free()
+    f64_flops = sum(float(ms[k]) * ns[k] * ns[k] for k in range(args.batch) if k not in split_set)
+    log(f"batch: {args.batch} problems, {info['graph_nodes']} graph nodes, {ms_step:.2f} ms per "
+        f"launch, {info['flops'] / (ms_step * 1e-3) * 1e-12:.2f} TFLOP/s-equivalent")
+    return {"problems": args.batch, "value": round(info["flops"] / (ms_step * 1e-3) * 1e-12, 4),
+            "unit": "TFLOP/s (sum m n^2 / launch time)", "ms_per_launch": round(ms_step, 4),
+            "flops": info["flops"], "fp64_share_of_flops": round(f64_flops / info["flops"], 4),
+            "fp64_tiles": info["fp64_tiles"], "graph_nodes": info["graph_nodes"],
+            "graph_build_s": round(build_s, 3), "tuning_table": bool(table)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +181,7 @@ def main():
     ap.add_argument("--spread", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-cols-per-rank", type=int, default=256)
+    ap.add_argument("--batch", type=int, default=512, help="problems of the batch leg (0 = skip)")
     ap.add_argument("--split-moduli", type=int, default=13,
                     help="moduli of the split-precision leg (0 = skip it)")
     args = ap.parse_args()
@@ -178,6 +225,8 @@ def main():
     if args.split_moduli > 0:
         split = split_leg(args, h, uvt, dA, dB, dC, flops, mean_ms)
 
+    batch = batch_leg(args, h, uvt) if args.batch > 0 else None
+
     cpu = None
     if not args.no_cpu:
         cols = min(m, 16 * args.cpu_cols_per_rank)
@@ -219,6 +268,7 @@ def main():
                      "algorithmic_flops_per_step": flops},
         "cpu_baseline": cpu,
         "split": split,
+        "batch": batch,
         "norm": norm,
     }
     print(json.dumps(out), flush=True)

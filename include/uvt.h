@@ -125,14 +125,58 @@ int32_t uvt_device_cus(uvt_handle h);
 uvt_status uvt_trmm_f64(uvt_handle h, uvt_plan plan, const double *dA, int64_t lda,
                         const double *dB, int64_t ldb, double *dC, int64_t ldc, void *stream);
 
-/* Split-precision C ~= tril(A) * B: the off-diagonal bulk as exact int8
- * slice products (int32 accumulation) recombined in FP64; the diagonal
- * blocks in FP64.  `slices` = number of int8 slices per operand (2..8);
- * more slices, smaller error.  workspace: see uvt_trmm_split_workspace. */
+/* Split-precision C ~= tril(A) * B: the box below each 256-row diagonal
+ * block as exact int8 products of residues modulo `slices` coprime moduli
+ * (2..14; int32 accumulation), recombined exactly (Chinese remainder) and
+ * rounded to FP64; the diagonal blocks in FP64.  More moduli, smaller
+ * error: 13 gives ~1e-13 relative at n = 32768.  dwork: 256-byte aligned
+ * device workspace of uvt_trmm_split_workspace bytes (n <= 131072). */
 uvt_status uvt_trmm_split_workspace(int64_t n, int64_t m, int32_t slices, size_t *bytes);
 uvt_status uvt_trmm_split(uvt_handle h, int64_t n, int64_t m, int32_t slices, const double *dA,
                           int64_t lda, const double *dB, int64_t ldb, double *dC, int64_t ldc,
                           void *dwork, size_t work_bytes, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Batched TRMMs (no counterpart in the reference).                    */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    const double *A; /* device, n x n lower, row-major */
+    int64_t lda;
+    const double *B; /* device, n x m */
+    int64_t ldb;
+    double *C;       /* device, n x m */
+    int64_t ldc;
+    int64_t n, m;
+    int32_t prec;    /* UVT_PREC_F64 or UVT_PREC_SPLIT */
+    int32_t reserved;
+} uvt_problem;
+
+typedef struct uvt_batch_s *uvt_batch;
+
+/* Plans a batch: the FP64 problems become one longest-first tile queue
+ * consumed by persistent kernels (tile shape per problem from the tuning
+ * table: a file written by uvt_autotune_f64, or NULL for the built-in one);
+ * the split problems run through batched split stages with `slices` moduli.
+ * Everything is captured once into a hipGraph on the handle's device.  Each
+ * problem's result does not depend on the other problems of the batch. */
+uvt_status uvt_batch_create(uvt_handle h, const uvt_problem *probs, int32_t count, int32_t slices,
+                            const char *tune_table, uvt_batch *out);
+/* One hipGraphLaunch of the whole batch on `stream`. */
+uvt_status uvt_batch_run(uvt_batch b, void *stream);
+/* New pointers for the same shapes and precisions (descriptors live in
+ * device memory, so the captured graph picks them up); synchronous. */
+uvt_status uvt_batch_update(uvt_batch b, const uvt_problem *probs, int32_t count);
+/* Algorithmic flops (sum of m n^2), FP64 tiles queued, graph nodes. */
+uvt_status uvt_batch_info(uvt_batch b, double *flops, int64_t *fp64_tiles, int32_t *graph_nodes);
+uvt_status uvt_batch_destroy(uvt_batch b);
+
+/* FP64 tile shapes selectable per problem. */
+typedef enum { UVT_TILE_128x128 = 0, UVT_TILE_64x64 = 1, UVT_TILE_128x32 = 2 } uvt_tile_cfg;
+/* Times every tile shape on (n, m) buckets (n 64..4096, m 16..1024) and
+ * writes the winners as a JSON table to out_path (synchronous; seconds). */
+uvt_status uvt_autotune_f64(uvt_handle h, const char *out_path);
+/* The tile shape the table (NULL: built-in) gives an n x n by n x m problem. */
+uvt_status uvt_tune_choice(const char *tune_table, int64_t n, int64_t m, int32_t *cfg);
 
 /* Row sums of |C| into d_rowsum[n] (device). */
 uvt_status uvt_row_abs_sums(uvt_handle h, int64_t n, int64_t m, const double *dC, int64_t ldc,

@@ -37,6 +37,15 @@ class Tile(_c.Structure):
     _fields_ = [(k, i32) for k in ("r0", "rows", "c0", "k_lo", "k_hi", "flags", "band", "cfg")]
 
 
+class Problem(_c.Structure):
+    _fields_ = [("A", vp), ("lda", i64), ("B", vp), ("ldb", i64), ("C", vp), ("ldc", i64),
+                ("n", i64), ("m", i64), ("prec", i32), ("reserved", i32)]
+
+
+PREC_F64, PREC_SPLIT = 0, 1
+TILE_CFGS = {0: "128x128", 1: "64x64", 2: "128x32"}
+
+
 class UvtError(RuntimeError):
     def __init__(self, status, what=""):
         self.status = status
@@ -67,6 +76,13 @@ _SIGS = {
     "uvt_trmm_f64": (i32, [vp, vp, vp, i64, vp, i64, vp, i64, vp]),
     "uvt_trmm_split_workspace": (i32, [i64, i64, i32, _c.POINTER(sz)]),
     "uvt_trmm_split": (i32, [vp, i64, i64, i32, vp, i64, vp, i64, vp, i64, vp, sz, vp]),
+    "uvt_batch_create": (i32, [vp, _c.c_void_p, i32, i32, _c.c_char_p, _c.POINTER(vp)]),
+    "uvt_batch_run": (i32, [vp, vp]),
+    "uvt_batch_update": (i32, [vp, _c.c_void_p, i32]),
+    "uvt_batch_info": (i32, [vp, _c.POINTER(f64), _c.POINTER(i64), _c.POINTER(i32)]),
+    "uvt_batch_destroy": (i32, [vp]),
+    "uvt_autotune_f64": (i32, [vp, _c.c_char_p]),
+    "uvt_tune_choice": (i32, [_c.c_char_p, i64, i64, _c.POINTER(i32)]),
     "uvt_row_abs_sums": (i32, [vp, i64, i64, vp, i64, vp, vp]),
     "uvt_inf_norm": (i32, [vp, i64, i64, vp, i64, _c.POINTER(f64), vp]),
     "uvt_fill_closed_form": (i32, [vp, i64, i64, vp, i64, vp, i64, vp]),
@@ -290,6 +306,51 @@ class Handle:
             lib().uvt_stream_destroy(self.ptr, self.stream)
             lib().uvt_destroy(self.ptr)
             self.ptr = None
+
+
+def tune_choice(n, m, table=None):
+    """Tile shape name the tuning table (None: built-in) picks for (n, m)."""
+    c = i32()
+    check(lib().uvt_tune_choice(table.encode() if table else None, n, m, _c.byref(c)), "tune_choice")
+    return TILE_CFGS[c.value]
+
+
+class Batch:
+    """Many TRMMs captured once into a hipGraph (see uvt_batch_create)."""
+
+    def __init__(self, handle, problems, slices=13, table=None):
+        self.h = handle
+        arr = (Problem * len(problems))(*problems)
+        self._arr = arr
+        b = vp()
+        check(lib().uvt_batch_create(handle.ptr, _c.cast(arr, _c.c_void_p), len(problems), slices,
+                                     table.encode() if table else None, _c.byref(b)), "batch_create")
+        self.ptr = b
+        self.count = len(problems)
+
+    def run(self, stream=None):
+        check(lib().uvt_batch_run(self.ptr, stream if stream is not None else self.h.stream),
+              "batch_run")
+
+    def update(self, problems):
+        arr = (Problem * len(problems))(*problems)
+        check(lib().uvt_batch_update(self.ptr, _c.cast(arr, _c.c_void_p), len(problems)),
+              "batch_update")
+        self._arr = arr
+
+    def info(self):
+        fl, tiles, nodes = f64(), i64(), i32()
+        check(lib().uvt_batch_info(self.ptr, _c.byref(fl), _c.byref(tiles), _c.byref(nodes)))
+        return {"flops": fl.value, "fp64_tiles": tiles.value, "graph_nodes": nodes.value}
+
+    def close(self):
+        if self.ptr:
+            lib().uvt_batch_destroy(self.ptr)
+            self.ptr = None
+
+
+def autotune(handle, path):
+    check(lib().uvt_autotune_f64(handle.ptr, path.encode()), "autotune")
 
 
 def gpu_available():

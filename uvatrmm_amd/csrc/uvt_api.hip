@@ -15,21 +15,14 @@ using namespace uvt;
 // Kernels
 // ---------------------------------------------------------------------
 
-// Workgroup shapes of the FP64 tile kernel: a 128x128 tile runs as 8 waves
-// of 32x64 (4 waves per SIMD with two workgroups per CU, which hides the
-// global-load latency better than 4 waves of 64x64); the 64x64 tile for
-// small problems runs as 4 waves of 32x32.
-template <int TM> struct TileWaves;
-template <> struct TileWaves<128> { static constexpr int M = 4, N = 2, BLOCKS_PER_CU = 2; };
-template <> struct TileWaves<64> { static constexpr int M = 2, N = 2, BLOCKS_PER_CU = 4; };
-
 template <int TM, int TN, bool VEC>
-__global__ __launch_bounds__(64 * TileWaves<TM>::M * TileWaves<TM>::N, TileWaves<TM>::BLOCKS_PER_CU)
+__global__ __launch_bounds__((64 * TileWaves<TM, TN>::M * TileWaves<TM, TN>::N),
+                             (TileWaves<TM, TN>::WAVES_PER_EU))
 void k_trmm_tiles_f64(const uvt_tile* __restrict__ tiles, const double* __restrict__ A, int64_t lda,
                       const double* __restrict__ B, int64_t ldb, double* __restrict__ C,
                       int64_t ldc, int64_t m)
 {
-    using W = TileWaves<TM>;
+    using W = TileWaves<TM, TN>;
     __shared__ double smem[F64Cfg<TM, TN, W::M, W::N>::SMEM_DOUBLES];
     const uvt_tile t = tiles[blockIdx.x];
     f64_tile<TM, TN, VEC, W::M, W::N>(t.r0, t.rows, t.c0, t.k_lo, t.k_hi, t.flags, A, lda, B, ldb,
@@ -222,7 +215,7 @@ static void launch_tiles(const uvt_tile* dtiles, int64_t ntiles, bool vec, const
                          int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc,
                          int64_t m, hipStream_t s)
 {
-    dim3 grid((unsigned)ntiles), block(64 * TileWaves<TM>::M * TileWaves<TM>::N);
+    dim3 grid((unsigned)ntiles), block(64 * TileWaves<TM, TN>::M * TileWaves<TM, TN>::N);
     if (vec)
         hipLaunchKernelGGL((k_trmm_tiles_f64<TM, TN, true>), grid, block, 0, s, dtiles, A, lda, B,
                            ldb, C, ldc, m);
@@ -241,6 +234,8 @@ uvt_status uvt_launch_tiles_f64(int32_t cfg, const uvt_tile* dtiles, int64_t nti
                      ldb % 2 == 0;
     if (cfg == UVT_CFG_64x64)
         launch_tiles<64, 64>(dtiles, ntiles, vec, A, lda, B, ldb, C, ldc, m, s);
+    else if (cfg == UVT_CFG_128x32)
+        launch_tiles<128, 32>(dtiles, ntiles, vec, A, lda, B, ldb, C, ldc, m, s);
     else
         launch_tiles<128, 128>(dtiles, ntiles, vec, A, lda, B, ldb, C, ldc, m, s);
     UVT_HIP(hipGetLastError());

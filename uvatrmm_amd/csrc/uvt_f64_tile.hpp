@@ -187,4 +187,22 @@ __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int
     }
 }
 
+// Workgroup shapes of the FP64 tile kernels (waves along rows x columns and
+// workgroups resident per CU).  A 128x128 tile runs as 8 waves of 32x64 (4
+// waves per SIMD with two workgroups per CU, which hides global-load latency
+// better than 4 waves of 64x64); 64x64 tiles (small problems) as 4 waves of
+// 32x32; 128x32 tiles (narrow B) as 4 waves of 32x32 stacked along rows.
+// WAVES_PER_EU (the second __launch_bounds__ argument on AMD) caps the
+// registers per wave so that BLOCKS_PER_CU workgroups fit on one CU.
+template <int TM, int TN> struct TileWaves;
+template <> struct TileWaves<128, 128> {
+    static constexpr int M = 4, N = 2, BLOCKS_PER_CU = 2, WAVES_PER_EU = 4;
+};
+template <> struct TileWaves<64, 64> {
+    static constexpr int M = 2, N = 2, BLOCKS_PER_CU = 3, WAVES_PER_EU = 3;
+};
+template <> struct TileWaves<128, 32> {
+    static constexpr int M = 4, N = 1, BLOCKS_PER_CU = 3, WAVES_PER_EU = 3;
+};
+
 }  // namespace uvt

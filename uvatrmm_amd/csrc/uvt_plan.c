@@ -34,7 +34,19 @@ const char *uvt_status_string(uvt_status s)
 const char *uvt_version(void) { return "uvatrmm-mi355x 0.1 (gfx950)"; }
 
 int32_t uvt_cfg_tile_m(int32_t cfg) { return cfg == UVT_CFG_64x64 ? 64 : 128; }
-int32_t uvt_cfg_tile_n(int32_t cfg) { return cfg == UVT_CFG_64x64 ? 64 : 128; }
+int32_t uvt_cfg_tile_n(int32_t cfg)
+{
+    return cfg == UVT_CFG_64x64 ? 64 : cfg == UVT_CFG_128x32 ? 32 : 128;
+}
+
+/* Narrow B: a 128x32 tile wastes no columns; small or narrow problems: 64x64
+ * tiles give enough workgroups; otherwise 128x128 (most reuse per load). */
+int32_t uvt_default_cfg(int64_t n, int64_t m)
+{
+    if (m <= 48) return UVT_CFG_128x32;
+    if (n <= 512 || m <= 96) return UVT_CFG_64x64;
+    return UVT_CFG_128x128;
+}
 
 /* ------------------------------------------------------------------ */
 /* Mapping                                                             */
@@ -251,7 +263,7 @@ uvt_status uvt_plan_create(int64_t n, int64_t m, uint32_t p, uvt_map map, uvt_sh
     pl->p = p;
     pl->map = map;
     pl->shape = shape;
-    pl->cfg = UVT_CFG_128x128;
+    pl->cfg = uvt_default_cfg(n, m);
     pl->dev_id = -1;
     pl->bands = (uvt_range *)calloc(p, sizeof(uvt_range));
     if (!pl->bands) {

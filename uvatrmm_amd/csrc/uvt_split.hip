@@ -16,19 +16,26 @@
 //   * For each modulus, the symmetric residues of A' and B' fit in int8; the
 //     product of residues with int32 accumulation is exact (|sum| < 2^29 for
 //     K < 32768 x 4), and is reduced to a residue of A'B' mod m_l.
-//   * Garner's algorithm turns the N residues into the mixed-radix digits of
-//     the unique C' in (-M/2, M/2); C' is assembled exactly in 128-bit integer
-//     arithmetic, converted to FP64, scaled by 2^-(alpha_i+beta_j) and added to
-//     the FP64 diagonal-block part.
+//   * The N residues give the mixed-radix digits of the unique C' in
+//     (-M/2, M/2) (Garner); C' is assembled exactly in two 64-bit words,
+//     rounded once to FP64, scaled by 2^-(alpha_i+beta_j) and added to the
+//     FP64 diagonal-block part.
 // The error comes only from rounding A and B to integers: about
 // sqrt(K)/||A'_i|| relative, i.e. ~1e-13 for 13 moduli at K = 32768.
+//
+// Every stage is a batched kernel over a list of problems: a workgroup finds
+// its problem from a per-stage lookup table (or directly when the batch has
+// one problem) and works on its local piece.  uvt_trmm_split is a batch of
+// one; the batched runner (uvt_batch.hip) sends many.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "uvt_f64_tile.hpp"
 #include "uvt_runtime.hpp"
+#include "uvt_split.hpp"
 
 using namespace uvt;
 
@@ -48,8 +55,8 @@ constexpr int kModuli[kMaxMods] = {256, 255, 253, 251, 247, 241, 239, 233,
 struct Mods {
     int n;
     int m[kMaxMods];
-    float invf[kMaxMods];
     double invd[kMaxMods];
+    float invf[kMaxMods];
     uint8_t prodmod[kMaxMods][kMaxMods];  // (m_0 ... m_{p-1}) mod m_l, p < l
     uint8_t qinv[kMaxMods];               // (m_0 ... m_{l-1})^-1 mod m_l
     uint32_t lift[kMaxMods];              // multiple of m_l >= 2^20 (keeps sums >= 0)
@@ -60,59 +67,95 @@ struct Mods {
     double wlo_hi, wlo_lo;                // wlo as a double-word
 };
 
+// Stages, in launch order.
+enum { K_DIAG = 0, K_ALPHA, K_BPART, K_BETA, K_RA, K_RB, K_GEMM, K_REC, K_NUM };
+
+// One problem as the kernels see it.
+struct Desc {
+    const double* A;
+    const double* B;
+    double* C;
+    int64_t lda, ldb, ldc;
+    int64_t n, m, kmax;       // kmax: deepest box (multiple of 256, 0 if n <= 256)
+    int* alpha;
+    int* beta;
+    double* part;
+    int8_t* Ar;               // [mod][n][kmax]
+    int8_t* Br;               // [mod][m][kmax] (B transposed)
+    uint8_t* R;               // [mod][n][m]
+    double target;
+    int32_t nrt, nct;         // 256-row/column tiles
+    int32_t nrt128, nct128;   // 128 tiles of the FP64 diagonal part
+    int32_t gxa, gxr;         // workgroups per row in resid_a / reconstruct
+    int32_t chunks;           // 256-row chunks of B's column norms
+    int32_t vec;              // 16-byte FP64 loads allowed
+    int32_t start[K_NUM];     // first workgroup of this problem in each stage
+    int32_t count[K_NUM];     // workgroups of this problem in each stage
+};
+
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
+
+// The problem of this workgroup (a copy in scalar registers: every field is
+// uniform over the workgroup).
+__device__ __forceinline__ Desc fetch(const Desc& one, const Desc* table, const int* lut, int nprob)
+{
+    return nprob == 1 ? one : table[lut[blockIdx.x]];
+}
 
 __device__ __forceinline__ int64_t row_box_end(int64_t i) { return (i / kTileI) * kTileI; }
 
 // ---- scales ---------------------------------------------------------------
 
 // alpha_i from the 2-norm of A[i][0:kb(i)); one wave per row.
-__global__ __launch_bounds__(256) void k_split_alpha(int64_t n, const double* __restrict__ A,
-                                                     int64_t lda, double target, int* __restrict__ alpha)
+__global__ __launch_bounds__(256) void k_alpha(Desc one, const Desc* table, const int* lut, int nprob)
 {
-    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const Desc d = fetch(one, table, lut, nprob);
+    const int64_t wb = (int64_t)blockIdx.x - d.start[K_ALPHA];
+    const int64_t row = wb * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (row >= n) return;
+    if (row >= d.n) return;
     const int64_t kb = row_box_end(row);
-    const double* a = A + row * lda;
+    const double* a = d.A + row * d.lda;
     double s = 0.0;
     for (int64_t k = lane; k < kb; k += 64) s = fma(a[k], a[k], s);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     if (lane == 0) {
         const double nrm = sqrt(s);
-        alpha[row] = nrm > 0.0 ? ilogb(target / nrm) : 0;
+        d.alpha[row] = nrm > 0.0 ? ilogb(d.target / nrm) : 0;
     }
 }
 
 // Partial column sums of B^2 over 256-row chunks of [0, kmax).
-__global__ __launch_bounds__(256) void k_split_beta_part(int64_t kmax, int64_t m,
-                                                         const double* __restrict__ B, int64_t ldb,
-                                                         double* __restrict__ part)
+__global__ __launch_bounds__(256) void k_beta_part(Desc one, const Desc* table, const int* lut, int nprob)
 {
-    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t k0 = (int64_t)blockIdx.y * 256;
-    if (j >= m) return;
-    const int64_t k1 = k0 + 256 < kmax ? k0 + 256 : kmax;
+    const Desc d = fetch(one, table, lut, nprob);
+    const int64_t wb = (int64_t)blockIdx.x - d.start[K_BPART];
+    const int64_t ncb = (d.m + 255) / 256;
+    const int64_t chunk = wb / ncb;
+    const int64_t j = (wb % ncb) * 256 + threadIdx.x;
+    if (j >= d.m) return;
+    const int64_t k0 = chunk * 256;
+    const int64_t k1 = k0 + 256 < d.kmax ? k0 + 256 : d.kmax;
     double s = 0.0;
     for (int64_t k = k0; k < k1; ++k) {
-        const double b = B[k * ldb + j];
+        const double b = d.B[k * d.ldb + j];
         s = fma(b, b, s);
     }
-    part[(int64_t)blockIdx.y * m + j] = s;
+    d.part[chunk * d.m + j] = s;
 }
 
-__global__ __launch_bounds__(256) void k_split_beta(int64_t m, int64_t chunks,
-                                                    const double* __restrict__ part, double target,
-                                                    int* __restrict__ beta)
+__global__ __launch_bounds__(256) void k_beta(Desc one, const Desc* table, const int* lut, int nprob)
 {
-    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (j >= m) return;
+    const Desc d = fetch(one, table, lut, nprob);
+    const int64_t wb = (int64_t)blockIdx.x - d.start[K_BETA];
+    const int64_t j = wb * 256 + threadIdx.x;
+    if (j >= d.m) return;
     double s = 0.0;
-    for (int64_t c = 0; c < chunks; ++c) s += part[c * m + j];
+    for (int64_t c = 0; c < d.chunks; ++c) s += d.part[c * d.m + j];
     const double nrm = sqrt(s);
-    beta[j] = nrm > 0.0 ? ilogb(target / nrm) : 0;
+    d.beta[j] = nrm > 0.0 ? ilogb(d.target / nrm) : 0;
 }
 
 // ---- residues ---------------------------------------------------------------
@@ -127,67 +170,68 @@ __device__ __forceinline__ int sym_residue(double a, int m, double invm)
     return r;
 }
 
-// Ar[l][i][k] for k < kb(i): 16 consecutive k per thread.
-__global__ __launch_bounds__(256) void k_split_resid_a(int64_t n, const double* __restrict__ A,
-                                                       int64_t lda, const int* __restrict__ alpha,
-                                                       int8_t* __restrict__ Ar, int64_t ldr,
-                                                       int64_t mod_stride, Mods mods)
+// Ar[l][i][k] for k < kb(i), rows i >= 256: a workgroup takes 4096 k of a row,
+// 16 consecutive k per thread.
+__global__ __launch_bounds__(256) void k_resid_a(Desc one, const Desc* table, const int* lut,
+                                                 int nprob, Mods mods)
 {
-    for (int64_t i = blockIdx.y; i < n; i += gridDim.y) {
-        const int64_t kb = row_box_end(i);
-        const int al = alpha[i];
-        for (int64_t k0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 16; k0 < kb;
-             k0 += (int64_t)gridDim.x * 256 * 16) {
-            double v[16];
-            const double* a = A + i * lda + k0;
+    const Desc d = fetch(one, table, lut, nprob);
+    const int64_t wb = (int64_t)blockIdx.x - d.start[K_RA];
+    const int64_t i = kTileI + wb / d.gxa;
+    const int64_t k0 = (wb % d.gxa) * 4096 + threadIdx.x * 16;
+    if (i >= d.n || k0 >= row_box_end(i)) return;  // kb(i) is a multiple of 16
+    const int al = d.alpha[i];
+    double v[16];
+    const double* a = d.A + i * d.lda + k0;
 #pragma unroll
-            for (int e = 0; e < 16; ++e) v[e] = rint(ldexp(a[e], al));
-            for (int l = 0; l < mods.n; ++l) {
-                uint32_t w[4];
+    for (int e = 0; e < 16; ++e) v[e] = rint(ldexp(a[e], al));
+    int8_t* out = d.Ar + i * d.kmax + k0;
+    const int64_t mod_stride = d.n * d.kmax;
+    for (int l = 0; l < mods.n; ++l) {
+        uint32_t q4[4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    uint32_t x = 0;
+        for (int q = 0; q < 4; ++q) {
+            uint32_t x = 0;
 #pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        x |= (uint32_t)(uint8_t)(int8_t)sym_residue(v[q * 4 + e], mods.m[l],
-                                                                     mods.invd[l])
-                             << (8 * e);
-                    w[q] = x;
-                }
-                *reinterpret_cast<v4i*>(Ar + l * mod_stride + i * ldr + k0) =
-                    v4i{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
-            }
+            for (int e = 0; e < 4; ++e)
+                x |= (uint32_t)(uint8_t)(int8_t)sym_residue(v[q * 4 + e], mods.m[l], mods.invd[l])
+                     << (8 * e);
+            q4[q] = x;
         }
+        *reinterpret_cast<v4i*>(out + l * mod_stride) =
+            v4i{(int)q4[0], (int)q4[1], (int)q4[2], (int)q4[3]};
     }
 }
 
 // Br[l][j][k] (B transposed) for k < kmax: 64 x 64 tiles through LDS.
-__global__ __launch_bounds__(256) void k_split_resid_b(int64_t kmax, int64_t m,
-                                                       const double* __restrict__ B, int64_t ldb,
-                                                       const int* __restrict__ beta,
-                                                       int8_t* __restrict__ Br, int64_t ldr,
-                                                       int64_t mod_stride, Mods mods)
+__global__ __launch_bounds__(256) void k_resid_b(Desc one, const Desc* table, const int* lut,
+                                                 int nprob, Mods mods)
 {
     __shared__ __attribute__((aligned(16))) int8_t t[64][64 + 16];
-    const int64_t k0 = (int64_t)blockIdx.x * 64, j0 = (int64_t)blockIdx.y * 64;
+    const Desc d = fetch(one, table, lut, nprob);
+    const int64_t wb = (int64_t)blockIdx.x - d.start[K_RB];
+    const int64_t nkx = d.kmax / 64;
+    const int64_t k0 = (wb % nkx) * 64, j0 = (wb / nkx) * 64;
     const int jj = threadIdx.x & 63, kq = threadIdx.x >> 6;
     const int64_t j = j0 + jj;
-    const bool jv = j < m;
-    const int be = jv ? beta[j] : 0;
+    const bool jv = j < d.m;
+    const int be = jv ? d.beta[j] : 0;
     double v[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-        const int64_t k = k0 + kq + 4 * q;  // kmax is a multiple of 256: k < kmax always
-        v[q] = jv ? rint(ldexp(B[k * ldb + j], be)) : 0.0;
+        const int64_t k = k0 + kq + 4 * q;  // < kmax: kmax is a multiple of 64
+        v[q] = jv ? rint(ldexp(d.B[k * d.ldb + j], be)) : 0.0;
     }
     const int wr = threadIdx.x >> 2, ws = (threadIdx.x & 3) * 16;
+    const int64_t mod_stride = d.m * d.kmax;
     for (int l = 0; l < mods.n; ++l) {
 #pragma unroll
-        for (int q = 0; q < 16; ++q) t[jj][kq + 4 * q] = (int8_t)sym_residue(v[q], mods.m[l], mods.invd[l]);
+        for (int q = 0; q < 16; ++q)
+            t[jj][kq + 4 * q] = (int8_t)sym_residue(v[q], mods.m[l], mods.invd[l]);
         __syncthreads();
-        if (j0 + wr < m) {
+        if (j0 + wr < d.m) {
             const v4i x = *reinterpret_cast<const v4i*>(&t[wr][ws]);
-            *reinterpret_cast<v4i*>(Br + l * mod_stride + (j0 + wr) * ldr + k0 + ws) = x;
+            *reinterpret_cast<v4i*>(d.Br + l * mod_stride + (j0 + wr) * d.kmax + k0 + ws) = x;
         }
         __syncthreads();
     }
@@ -261,28 +305,35 @@ __device__ __forceinline__ void i8_tile_loop(const int8_t* __restrict__ A, int64
 }
 
 // One workgroup: a 256 x 256 tile of (A' B') mod m_l over k in [0, r0).
-__global__ __launch_bounds__(512, 1) void k_split_gemm_i8(
-    const int8_t* __restrict__ Ar, int64_t lda_r, int64_t a_mod_stride,
-    const int8_t* __restrict__ Br, int64_t ldb_r, int64_t b_mod_stride,
-    uint8_t* __restrict__ R, int64_t ldr, int64_t r_mod_stride, int64_t n, int64_t m, int nrt,
-    int nct, Mods mods)
+__global__ __launch_bounds__(512, 1) void k_gemm_i8(Desc one, const Desc* table, const int* lut,
+                                                    int nprob, Mods mods)
 {
     __shared__ __attribute__((aligned(16))) int8_t smem[2 * kI8Stage];
-    // XCD-aware mapping.  Workgroups go to the 8 XCDs round-robin, so XCD x
-    // runs blocks x, x+8, ...; its s-th block takes tile s%32 of tile group
+    // Only the fields used here are read (from the kernel argument or the
+    // descriptor table), so the descriptor does not occupy registers.
+    const Desc* dp = nprob == 1 ? nullptr : &table[lut[blockIdx.x]];
+#define UVT_FIELD(f) (dp ? dp->f : one.f)
+    const int64_t wb = (int64_t)blockIdx.x - UVT_FIELD(start[K_GEMM]);
+    // XCD-aware mapping.  Workgroups go to the 8 XCDs round-robin (every
+    // problem starts at a multiple of 256 workgroups), so XCD x runs local
+    // blocks x, x+8, ...; its s-th block takes tile s%32 of tile group
     // (s/32)*8 + x.  A group is 4 row tiles x 8 column tiles of one modulus,
     // i.e. the 32 workgroups one XCD holds at once, so they share 4 A panels
     // and 8 B panels through that XCD's L2.  Groups run longest-k first.
-    const int xcd = (int)(blockIdx.x & 7), slot = (int)(blockIdx.x >> 3);
+    const int xcd = (int)(wb & 7), slot = (int)(wb >> 3);
     const int g = (slot >> 5) * 8 + xcd, t = slot & 31;
+    const int nct = UVT_FIELD(nct), nrt = UVT_FIELD(nrt);
     const int ngc = (nct + 7) / 8, per_rg = ngc * mods.n;
     const int rg = g / per_rg, grem = g % per_rg;
     const int l = grem / ngc, ct = (grem % ngc) * 8 + (t & 7);
     const int rt = nrt - 1 - (rg * 4 + (t >> 3));
     if (rt < 1 || ct >= nct) return;  // padding of the last groups (uniform per block)
     const int64_t r0 = (int64_t)rt * kTileI, c0 = (int64_t)ct * kTileI;
-    const int8_t* A = Ar + l * a_mod_stride + r0 * lda_r;
-    const int8_t* B = Br + l * b_mod_stride + c0 * ldb_r;
+    const int64_t n = UVT_FIELD(n), m = UVT_FIELD(m), kmax = UVT_FIELD(kmax);
+    const int8_t* A = UVT_FIELD(Ar) + l * (n * kmax) + r0 * kmax;
+    const int8_t* B = UVT_FIELD(Br) + l * (m * kmax) + c0 * kmax;
+    uint8_t* const Rout = UVT_FIELD(R);
+#undef UVT_FIELD
 
     v16i acc[2][4];
 #pragma unroll
@@ -290,9 +341,9 @@ __global__ __launch_bounds__(512, 1) void k_split_gemm_i8(
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = v16i{0};
     if (r0 + kTileI <= n && c0 + kTileI <= m)
-        i8_tile_loop<true>(A, lda_r, B, ldb_r, n - r0, m - c0, (int)r0, smem, acc);
+        i8_tile_loop<true>(A, kmax, B, kmax, n - r0, m - c0, (int)r0, smem, acc);
     else
-        i8_tile_loop<false>(A, lda_r, B, ldb_r, n - r0, m - c0, (int)r0, smem, acc);
+        i8_tile_loop<false>(A, kmax, B, kmax, n - r0, m - c0, (int)r0, smem, acc);
 
     // D layout of v_mfma_i32_32x32x32_i8 (measured on gfx950): lane l, element
     // v holds row 8*(v/4) + 4*(l/32) + v%4, column l%32.
@@ -300,7 +351,7 @@ __global__ __launch_bounds__(512, 1) void k_split_gemm_i8(
     const int wm = wave >> 1, wn = wave & 1;
     const int mod = mods.m[l];
     const float invf = mods.invf[l];
-    uint8_t* out = R + l * r_mod_stride;
+    uint8_t* out = Rout + l * (n * m);
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -314,7 +365,7 @@ __global__ __launch_bounds__(512, 1) void k_split_gemm_i8(
                     int r = x - mod * (int)floorf((float)x * invf);
                     if (r < 0) r += mod;
                     if (r >= mod) r -= mod;
-                    out[row * ldr + col] = (uint8_t)r;
+                    out[row * m + col] = (uint8_t)r;
                 }
             }
         }
@@ -324,22 +375,27 @@ __global__ __launch_bounds__(512, 1) void k_split_gemm_i8(
 
 // Rows [r0, r0+128) of C = A[rows, t0:row] * B[t0:row] where t0 is the start of
 // the 256-row tile holding r0 (lower mask), written (not accumulated).
-__global__ __launch_bounds__(512, 2) void k_split_diag_f64(int64_t n, int64_t m, int nct,
-                                                           const double* __restrict__ A, int64_t lda,
-                                                           const double* __restrict__ B, int64_t ldb,
-                                                           double* __restrict__ C, int64_t ldc, int vec)
+__global__ __launch_bounds__(512, 4) void k_diag_f64(Desc one, const Desc* table, const int* lut, int nprob)
 {
     __shared__ double smem[F64Cfg<128, 128, 4, 2>::SMEM_DOUBLES];
-    const int rt = (int)(blockIdx.x / nct), ct = (int)(blockIdx.x % nct);
+    const Desc* dp = nprob == 1 ? nullptr : &table[lut[blockIdx.x]];
+#define UVT_FIELD(f) (dp ? dp->f : one.f)
+    const int64_t wb = (int64_t)blockIdx.x - UVT_FIELD(start[K_DIAG]);
+    const int nct128 = UVT_FIELD(nct128);
+    const int64_t n = UVT_FIELD(n);
+    const int rt = (int)(wb / nct128), ct = (int)(wb % nct128);
     const int r0 = rt * 128, c0 = ct * 128;
     const int rows = (int)(n - r0 < 128 ? n - r0 : 128);
     const int k_lo = (r0 / kTileI) * kTileI;
-    if (vec)
-        f64_tile<128, 128, true, 4, 2>(r0, rows, c0, k_lo, r0 + rows, kTileLower, A, lda, B, ldb, C,
-                                       ldc, m, smem);
+    if (UVT_FIELD(vec))
+        f64_tile<128, 128, true, 4, 2>(r0, rows, c0, k_lo, r0 + rows, kTileLower, UVT_FIELD(A),
+                                       UVT_FIELD(lda), UVT_FIELD(B), UVT_FIELD(ldb), UVT_FIELD(C),
+                                       UVT_FIELD(ldc), UVT_FIELD(m), smem);
     else
-        f64_tile<128, 128, false, 4, 2>(r0, rows, c0, k_lo, r0 + rows, kTileLower, A, lda, B, ldb,
-                                        C, ldc, m, smem);
+        f64_tile<128, 128, false, 4, 2>(r0, rows, c0, k_lo, r0 + rows, kTileLower, UVT_FIELD(A),
+                                        UVT_FIELD(lda), UVT_FIELD(B), UVT_FIELD(ldb), UVT_FIELD(C),
+                                        UVT_FIELD(ldc), UVT_FIELD(m), smem);
+#undef UVT_FIELD
 }
 
 // ---- reconstruction -----------------------------------------------------------
@@ -357,104 +413,80 @@ __device__ __forceinline__ uint32_t mod_magic(uint32_t x, uint32_t m, uint32_t m
 //   v_l = (r_l - sum_{p<l} v_p (m_0...m_{p-1} mod m_l)) (m_0...m_{l-1})^-1 mod m_l,
 // then x = X_lo + W X_hi with X_lo = digits [0, split) and X_hi = the rest,
 // both exact in 64 bits, and the signed value (x, or x - M when x > M/2)
-// rounded once to FP64 with a double-word W.  Four columns per thread.
-template <bool VEC4>
-__global__ __launch_bounds__(256) void k_split_reconstruct(int64_t n, int64_t m,
-                                                           const uint8_t* __restrict__ R, int64_t ldr,
-                                                           int64_t r_mod_stride,
-                                                           const int* __restrict__ alpha,
-                                                           const int* __restrict__ beta,
-                                                           double* __restrict__ C, int64_t ldc,
-                                                           Mods mods)
+// rounded once to FP64 with a double-word W.  A workgroup takes 1024 columns
+// of a row, four per thread.
+__global__ __launch_bounds__(256) void k_reconstruct(Desc one, const Desc* table, const int* lut,
+                                                     int nprob, Mods mods)
 {
-    for (int64_t i = kTileI + blockIdx.y; i < n; i += gridDim.y)
-    for (int64_t j4 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 4; j4 < m;
-         j4 += (int64_t)gridDim.x * 256 * 4) {
-        const int ne = m - j4 < 4 ? (int)(m - j4) : 4;
-        uint32_t res[kMaxMods][4];
+    const Desc d = fetch(one, table, lut, nprob);
+    const int64_t wb = (int64_t)blockIdx.x - d.start[K_REC];
+    const int64_t i = kTileI + wb / d.gxr;
+    const int64_t j4 = (wb % d.gxr) * 1024 + threadIdx.x * 4;
+    const int64_t m = d.m;
+    if (i >= d.n || j4 >= m) return;
+    const int ne = m - j4 < 4 ? (int)(m - j4) : 4;
+    const bool vec4 = (m & 3) == 0 && ne == 4;
+    const int64_t mod_stride = d.n * m;
+    uint32_t res[kMaxMods][4];
+#pragma unroll
+    for (int l = 0; l < kMaxMods; ++l) {
+        if (l < mods.n) {
+            const uint8_t* r = d.R + l * mod_stride + i * m + j4;
+            if (vec4) {
+                const uint32_t x = *reinterpret_cast<const uint32_t*>(r);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) res[l][e] = (x >> (8 * e)) & 255u;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) res[l][e] = e < ne ? r[e] : 0u;
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) res[l][e] = 0u;
+        }
+    }
+    const int ai = d.alpha[i];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        if (e >= ne) break;
+        uint32_t v[kMaxMods];
 #pragma unroll
         for (int l = 0; l < kMaxMods; ++l) {
+            uint32_t t = 0;
             if (l < mods.n) {
-                const uint8_t* r = R + l * r_mod_stride + i * ldr + j4;
-                if (VEC4 && ne == 4) {
-                    const uint32_t w = *reinterpret_cast<const uint32_t*>(r);
+                uint32_t acc = mods.lift[l] + res[l][e];
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) res[l][e] = (w >> (8 * e)) & 255u;
-                } else {
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) res[l][e] = e < ne ? r[e] : 0u;
-                }
-            } else {
-#pragma unroll
-                for (int e = 0; e < 4; ++e) res[l][e] = 0u;
+                for (int p = 0; p < l; ++p) acc -= v[p] * (uint32_t)mods.prodmod[p][l];
+                t = mod_magic(acc, (uint32_t)mods.m[l], mods.magic[l]);
+                t = mod_magic(t * (uint32_t)mods.qinv[l], (uint32_t)mods.m[l], mods.magic[l]);
             }
+            v[l] = t;
         }
-        const int ai = alpha[i];
+        uint64_t xlo = 0, xhi = 0;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            if (e >= ne) break;
-            uint32_t v[kMaxMods];
-#pragma unroll
-            for (int l = 0; l < kMaxMods; ++l) {
-                uint32_t t = 0;
-                if (l < mods.n) {
-                    uint32_t acc = mods.lift[l] + res[l][e];
-#pragma unroll
-                    for (int p = 0; p < l; ++p) acc -= v[p] * (uint32_t)mods.prodmod[p][l];
-                    t = mod_magic(acc, (uint32_t)mods.m[l], mods.magic[l]);
-                    t = mod_magic(t * (uint32_t)mods.qinv[l], (uint32_t)mods.m[l], mods.magic[l]);
-                }
-                v[l] = t;
-            }
-            uint64_t xlo = 0, xhi = 0;
-#pragma unroll
-            for (int l = kMaxMods - 1; l >= 0; --l) {
-                if (l < mods.split) xlo = xlo * (uint64_t)mods.m[l] + v[l];
-                else if (l < mods.n) xhi = xhi * (uint64_t)mods.m[l] + v[l];
-            }
-            double h, low;
-            if (2 * xhi > mods.qhi) {  // x > M/2: value x - M = (X_hi - Q + 1) W + (X_lo - W)
-                h = (double)((int64_t)xhi - (int64_t)mods.qhi + 1);
-                low = (double)((int64_t)xlo - (int64_t)mods.wlo);
-            } else {
-                h = (double)xhi;
-                low = (double)xlo;
-            }
-            const double d = fma(h, mods.wlo_hi, fma(h, mods.wlo_lo, low));
-            const int64_t j = j4 + e;
-            C[i * ldc + j] += ldexp(d, -(ai + beta[j]));
+        for (int l = kMaxMods - 1; l >= 0; --l) {
+            if (l < mods.split) xlo = xlo * (uint64_t)mods.m[l] + v[l];
+            else if (l < mods.n) xhi = xhi * (uint64_t)mods.m[l] + v[l];
         }
+        double h, low;
+        if (2 * xhi > mods.qhi) {  // x > M/2: value x - M = (X_hi - Q + 1) W + (X_lo - W)
+            h = (double)((int64_t)xhi - (int64_t)mods.qhi + 1);
+            low = (double)((int64_t)xlo - (int64_t)mods.wlo);
+        } else {
+            h = (double)xhi;
+            low = (double)xlo;
+        }
+        const double x = fma(h, mods.wlo_hi, fma(h, mods.wlo_lo, low));
+        const int64_t j = j4 + e;
+        d.C[i * d.ldc + j] += ldexp(x, -(ai + d.beta[j]));
     }
 }
 
 // ---- host side ------------------------------------------------------------------
 
-struct Layout {
-    int64_t kmax, lda_r, ldb_r, chunks;
-    size_t off_alpha, off_beta, off_part, off_ar, off_br, off_r, total;
-};
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
-static Layout layout(int64_t n, int64_t m, int nmod)
-{
-    Layout L;
-    L.kmax = ((n - 1) / kTileI) * kTileI;  // largest box depth (multiple of 256)
-    L.lda_r = L.kmax;
-    L.ldb_r = L.kmax;
-    L.chunks = (L.kmax + 255) / 256;
-    size_t o = 0;
-    L.off_alpha = o; o = align256(o + (size_t)n * 4);
-    L.off_beta = o;  o = align256(o + (size_t)m * 4);
-    L.off_part = o;  o = align256(o + (size_t)(L.chunks ? L.chunks : 1) * m * 8);
-    L.off_ar = o;    o = align256(o + (size_t)nmod * n * L.lda_r);
-    L.off_br = o;    o = align256(o + (size_t)nmod * m * L.ldb_r);
-    L.off_r = o;     o = align256(o + (size_t)nmod * n * m);
-    L.total = o;
-    return L;
-}
-
-static int inv_mod(int a, int m)
+int inv_mod(int a, int m)
 {
     a %= m;
     for (int x = 1; x < m; ++x)
@@ -462,7 +494,7 @@ static int inv_mod(int a, int m)
     return 0;
 }
 
-static Mods make_mods(int nmod, double* log2M)
+Mods make_mods(int nmod, double* log2M)
 {
     Mods md;
     memset(&md, 0, sizeof md);
@@ -495,13 +527,197 @@ static Mods make_mods(int nmod, double* log2M)
     return md;
 }
 
+// Workspace bytes of one problem (scales, partial norms, residues).
+size_t problem_bytes(int64_t n, int64_t m, int nmod)
+{
+    const int64_t kmax = ((n - 1) / kTileI) * kTileI;
+    if (kmax == 0) return 0;
+    const int64_t chunks = kmax / 256;
+    size_t o = 0;
+    o = align256(o + (size_t)n * 4);
+    o = align256(o + (size_t)m * 4);
+    o = align256(o + (size_t)chunks * m * 8);
+    o = align256(o + (size_t)nmod * n * kmax);
+    o = align256(o + (size_t)nmod * m * kmax);
+    o = align256(o + (size_t)nmod * n * m);
+    return o;
+}
+
+uvt_status check_problem(const uvt_split_problem& p)
+{
+    if (!p.A || !p.B || !p.C || p.n <= 0 || p.m <= 0 || p.lda < p.n || p.ldb < p.m || p.ldc < p.m)
+        return UVT_ERR_ARG;
+    if (p.n > 131072 || p.m > 0x3FFFFFFF) return UVT_ERR_SIZE;  // int32 accumulation bound on k
+    return UVT_OK;
+}
+
+// Fills the descriptors (workspace pointers from `work` on) and the stage
+// workgroup counts; returns the per-stage totals.
+void describe(const uvt_split_problem* probs, int count, int nmod, char* work, Desc* out,
+              int64_t totals[K_NUM])
+{
+    double log2M = 0.0;
+    make_mods(nmod, &log2M);
+    double lt = (log2M - 2.0) / 2.0;  // |A'_i| |B'_j| <= M/4 and every scaled entry < 2^53
+    if (lt > 52.0) lt = 52.0;
+    for (int k = 0; k < K_NUM; ++k) totals[k] = 0;
+    size_t off = 0;
+    for (int p = 0; p < count; ++p) {
+        const uvt_split_problem& q = probs[p];
+        Desc& d = out[p];
+        memset(&d, 0, sizeof d);
+        d.A = q.A;
+        d.B = q.B;
+        d.C = q.C;
+        d.lda = q.lda;
+        d.ldb = q.ldb;
+        d.ldc = q.ldc;
+        d.n = q.n;
+        d.m = q.m;
+        d.kmax = ((q.n - 1) / kTileI) * kTileI;
+        d.target = ldexp(1.0, (int)floor(lt)) - 0.5 * sqrt((double)(d.kmax + 1));
+        d.nrt = (int32_t)((q.n + kTileI - 1) / kTileI);
+        d.nct = (int32_t)((q.m + kTileI - 1) / kTileI);
+        d.nrt128 = (int32_t)((q.n + 127) / 128);
+        d.nct128 = (int32_t)((q.m + 127) / 128);
+        d.gxa = (int32_t)((d.kmax + 4095) / 4096);
+        d.gxr = (int32_t)((q.m + 1023) / 1024);
+        d.chunks = (int32_t)(d.kmax / 256);
+        d.vec = ((uintptr_t)q.A % 16 == 0) && ((uintptr_t)q.B % 16 == 0) && q.lda % 2 == 0 &&
+                q.ldb % 2 == 0;
+        if (d.kmax > 0) {
+            char* w = work ? work + off : nullptr;
+            size_t o = 0;
+            d.alpha = (int*)(w + o);
+            o = align256(o + (size_t)q.n * 4);
+            d.beta = (int*)(w + o);
+            o = align256(o + (size_t)q.m * 4);
+            d.part = (double*)(w + o);
+            o = align256(o + (size_t)d.chunks * q.m * 8);
+            d.Ar = (int8_t*)(w + o);
+            o = align256(o + (size_t)nmod * q.n * d.kmax);
+            d.Br = (int8_t*)(w + o);
+            o = align256(o + (size_t)nmod * q.m * d.kmax);
+            d.R = (uint8_t*)(w + o);
+            o = align256(o + (size_t)nmod * q.n * q.m);
+            off += o;
+            d.count[K_ALPHA] = (int32_t)((q.n + 3) / 4);
+            d.count[K_BPART] = (int32_t)(((q.m + 255) / 256) * d.chunks);
+            d.count[K_BETA] = (int32_t)((q.m + 255) / 256);
+            d.count[K_RA] = (int32_t)((q.n - kTileI) * d.gxa);
+            d.count[K_RB] = (int32_t)((d.kmax / 64) * ((q.m + 63) / 64));
+            const int64_t groups = (int64_t)((d.nrt - 1 + 3) / 4) * ((d.nct + 7) / 8) * nmod;
+            d.count[K_GEMM] = (int32_t)((groups + 7) / 8 * 8 * 32);  // multiple of 256
+            d.count[K_REC] = (int32_t)((q.n - kTileI) * d.gxr);
+        }
+        d.count[K_DIAG] = d.nrt128 * d.nct128;
+        for (int k = 0; k < K_NUM; ++k) {
+            d.start[k] = (int32_t)totals[k];
+            totals[k] += d.count[k];
+        }
+    }
+}
+
 }  // namespace
+
+// ---- batch plumbing shared with uvt_batch.hip -----------------------------------
+
+size_t uvt_split_desc_bytes(int count) { return align256((size_t)count * sizeof(Desc)); }
+
+uvt_status uvt_split_batch_sizes(const uvt_split_problem* probs, int count, int nmod,
+                                 size_t* work_bytes, int64_t* lut_entries)
+{
+    if (nmod < 2 || nmod > kMaxSlices || count < 0) return UVT_ERR_ARG;
+    size_t w = 0;
+    for (int p = 0; p < count; ++p) {
+        uvt_status st = check_problem(probs[p]);
+        if (st != UVT_OK) return st;
+        w += problem_bytes(probs[p].n, probs[p].m, nmod);
+    }
+    int64_t lut = 0;
+    if (count > 1) {
+        std::vector<Desc> d(count);
+        int64_t tot[K_NUM];
+        describe(probs, count, nmod, nullptr, d.data(), tot);
+        for (int k = 0; k < K_NUM; ++k) {
+            if (tot[k] > 0x7FFFFFFF) return UVT_ERR_SIZE;
+            lut += tot[k];
+        }
+    }
+    *work_bytes = w;
+    *lut_entries = lut;
+    return UVT_OK;
+}
+
+uvt_status uvt_split_batch_build(const uvt_split_problem* probs, int count, int nmod, void* dwork,
+                                 std::vector<char>& desc_img, std::vector<int>& lut_img,
+                                 int64_t totals[8], int64_t lut_off[8])
+{
+    std::vector<Desc> d(count);
+    describe(probs, count, nmod, (char*)dwork, d.data(), totals);
+    desc_img.assign((const char*)d.data(), (const char*)d.data() + count * sizeof(Desc));
+    lut_img.clear();
+    for (int k = 0; k < K_NUM; ++k) {
+        lut_off[k] = (int64_t)lut_img.size();
+        if (count <= 1) continue;
+        for (int p = 0; p < count; ++p)
+            for (int32_t b = 0; b < d[p].count[k]; ++b) lut_img.push_back(p);
+    }
+    return UVT_OK;
+}
+
+uvt_status uvt_split_batch_launch(const uvt_split_problem* single, int count, int nmod,
+                                  void* dwork, const void* ddesc, const int* dlut,
+                                  const int64_t totals_in[8], const int64_t lut_off[8],
+                                  hipStream_t s, int stages)
+{
+    double log2M = 0.0;
+    const Mods mods = make_mods(nmod, &log2M);
+    Desc one;
+    memset(&one, 0, sizeof one);
+    int64_t totals[K_NUM];
+    if (count == 1) {
+        describe(single, 1, nmod, (char*)dwork, &one, totals);
+    } else {
+        for (int k = 0; k < K_NUM; ++k) totals[k] = totals_in[k];
+    }
+    const Desc* table = (const Desc*)ddesc;
+    auto lut = [&](int k) { return count == 1 ? (const int*)nullptr : dlut + lut_off[k]; };
+    // The FP64 diagonal part stores every row of C first; the box part adds to it.
+    if ((stages & UVT_SPLIT_DIAG) && totals[K_DIAG])
+        hipLaunchKernelGGL(k_diag_f64, dim3((unsigned)totals[K_DIAG]), dim3(512), 0, s, one, table,
+                           lut(K_DIAG), count);
+    if ((stages & UVT_SPLIT_BOX) && totals[K_ALPHA])
+        hipLaunchKernelGGL(k_alpha, dim3((unsigned)totals[K_ALPHA]), dim3(256), 0, s, one, table,
+                           lut(K_ALPHA), count);
+    if ((stages & UVT_SPLIT_BOX) && totals[K_BPART])
+        hipLaunchKernelGGL(k_beta_part, dim3((unsigned)totals[K_BPART]), dim3(256), 0, s, one, table,
+                           lut(K_BPART), count);
+    if ((stages & UVT_SPLIT_BOX) && totals[K_BETA])
+        hipLaunchKernelGGL(k_beta, dim3((unsigned)totals[K_BETA]), dim3(256), 0, s, one, table,
+                           lut(K_BETA), count);
+    if ((stages & UVT_SPLIT_BOX) && totals[K_RA])
+        hipLaunchKernelGGL(k_resid_a, dim3((unsigned)totals[K_RA]), dim3(256), 0, s, one, table,
+                           lut(K_RA), count, mods);
+    if ((stages & UVT_SPLIT_BOX) && totals[K_RB])
+        hipLaunchKernelGGL(k_resid_b, dim3((unsigned)totals[K_RB]), dim3(256), 0, s, one, table,
+                           lut(K_RB), count, mods);
+    if ((stages & UVT_SPLIT_BOX) && totals[K_GEMM])
+        hipLaunchKernelGGL(k_gemm_i8, dim3((unsigned)totals[K_GEMM]), dim3(512), 0, s, one, table,
+                           lut(K_GEMM), count, mods);
+    if ((stages & UVT_SPLIT_REC) && totals[K_REC])
+        hipLaunchKernelGGL(k_reconstruct, dim3((unsigned)totals[K_REC]), dim3(256), 0, s, one, table,
+                           lut(K_REC), count, mods);
+    UVT_HIP(hipGetLastError());
+    return UVT_OK;
+}
 
 extern "C" uvt_status uvt_trmm_split_workspace(int64_t n, int64_t m, int32_t slices, size_t* bytes)
 {
     if (!bytes || n <= 0 || m <= 0 || slices < 2 || slices > kMaxSlices) return UVT_ERR_ARG;
     if (n > 131072 || m > 0x3FFFFFFF) return UVT_ERR_SIZE;  // int32 accumulation bound on k
-    *bytes = layout(n, m, slices).total;
+    *bytes = problem_bytes(n, m, slices);
+    if (*bytes == 0) *bytes = 256;
     return UVT_OK;
 }
 
@@ -510,74 +726,16 @@ extern "C" uvt_status uvt_trmm_split(uvt_handle h, int64_t n, int64_t m, int32_t
                                      double* dC, int64_t ldc, void* dwork, size_t work_bytes,
                                      void* stream)
 {
-    if (!h || !dA || !dB || !dC || n <= 0 || m <= 0 || lda < n || ldb < m || ldc < m)
-        return UVT_ERR_ARG;
+    if (!h) return UVT_ERR_ARG;
+    const uvt_split_problem p = {dA, lda, dB, ldb, dC, ldc, n, m};
+    uvt_status st = check_problem(p);
+    if (st != UVT_OK) return st;
     size_t need = 0;
-    uvt_status st = uvt_trmm_split_workspace(n, m, slices, &need);
+    st = uvt_trmm_split_workspace(n, m, slices, &need);
     if (st != UVT_OK) return st;
     if (!dwork || work_bytes < need || ((uintptr_t)dwork & 255)) return UVT_ERR_ARG;
     UVT_HIP(hipSetDevice(h->device));
-    hipStream_t s = (hipStream_t)stream;
-    const Layout L = layout(n, m, slices);
-    double log2M = 0.0;
-    const Mods mods = make_mods(slices, &log2M);
-    // |A'_i| |B'_j| must stay below M/2 and every scaled entry below 2^53.
-    double lt = (log2M - 2.0) / 2.0;
-    if (lt > 52.0) lt = 52.0;
-    const double target = ldexp(1.0, (int)floor(lt)) - 0.5 * sqrt((double)(L.kmax + 1));
-    char* w = (char*)dwork;
-    int* alpha = (int*)(w + L.off_alpha);
-    int* beta = (int*)(w + L.off_beta);
-    double* part = (double*)(w + L.off_part);
-    int8_t* Ar = (int8_t*)(w + L.off_ar);
-    int8_t* Br = (int8_t*)(w + L.off_br);
-    uint8_t* R = (uint8_t*)(w + L.off_r);
-    const bool vec = ((uintptr_t)dA % 16 == 0) && ((uintptr_t)dB % 16 == 0) && lda % 2 == 0 &&
-                     ldb % 2 == 0;
-
-    // FP64 diagonal blocks (store every row of C).
-    const int nct128 = (int)((m + 127) / 128), nrt128 = (int)((n + 127) / 128);
-    hipLaunchKernelGGL(k_split_diag_f64, dim3((unsigned)(nrt128 * nct128)), dim3(512), 0, s, n, m,
-                       nct128, dA, lda, dB, ldb, dC, ldc, vec ? 1 : 0);
-    UVT_HIP(hipGetLastError());
-    if (L.kmax == 0) return UVT_OK;  // n <= 256: no box part
-
-    hipLaunchKernelGGL(k_split_alpha, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, n, dA, lda,
-                       target, alpha);
-    hipLaunchKernelGGL(k_split_beta_part, dim3((unsigned)((m + 255) / 256), (unsigned)L.chunks),
-                       dim3(256), 0, s, L.kmax, m, dB, ldb, part);
-    hipLaunchKernelGGL(k_split_beta, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, m,
-                       L.chunks, part, target, beta);
-    {
-        int64_t gx = (L.kmax / 16 + 255) / 256;
-        if (gx > 64) gx = 64;
-        const int64_t gy = n < 16384 ? n : 16384;
-        hipLaunchKernelGGL(k_split_resid_a, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, s, n, dA,
-                           lda, alpha, Ar, L.lda_r, (int64_t)n * L.lda_r, mods);
-    }
-    hipLaunchKernelGGL(k_split_resid_b, dim3((unsigned)(L.kmax / 64), (unsigned)((m + 63) / 64)),
-                       dim3(256), 0, s, L.kmax, m, dB, ldb, beta, Br, L.ldb_r,
-                       (int64_t)m * L.ldb_r, mods);
-    UVT_HIP(hipGetLastError());
-    const int nrt = (int)((n + kTileI - 1) / kTileI), nct = (int)((m + kTileI - 1) / kTileI);
-    const int64_t groups = (int64_t)((nrt - 1 + 3) / 4) * ((nct + 7) / 8) * slices;
-    const int64_t blocks = (groups + 7) / 8 * 8 * 32;
-    if (blocks > 0x7FFFFFFF) return UVT_ERR_SIZE;
-    hipLaunchKernelGGL(k_split_gemm_i8, dim3((unsigned)blocks), dim3(512), 0, s, Ar, L.lda_r,
-                       (int64_t)n * L.lda_r, Br, L.ldb_r, (int64_t)m * L.ldb_r, R, m, n * m, n, m, nrt,
-                       nct, mods);
-    UVT_HIP(hipGetLastError());
-    {
-        int64_t gx = (m + 1023) / 1024;
-        if (gx > 32) gx = 32;
-        const int64_t gy = n - kTileI < 32768 ? n - kTileI : 32768;
-        if (m % 4 == 0)
-            hipLaunchKernelGGL(k_split_reconstruct<true>, dim3((unsigned)gx, (unsigned)gy), dim3(256),
-                               0, s, n, m, R, m, n * m, alpha, beta, dC, ldc, mods);
-        else
-            hipLaunchKernelGGL(k_split_reconstruct<false>, dim3((unsigned)gx, (unsigned)gy), dim3(256),
-                               0, s, n, m, R, m, n * m, alpha, beta, dC, ldc, mods);
-    }
-    UVT_HIP(hipGetLastError());
-    return UVT_OK;
+    const int64_t zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    return uvt_split_batch_launch(&p, 1, slices, dwork, nullptr, nullptr, zero, zero,
+                                  (hipStream_t)stream, UVT_SPLIT_ALL);
 }
