@@ -134,10 +134,11 @@ def batch_leg(args, h, uvt):
     bufs, probs = [], []
     for k in range(args.batch):
         n, m = int(ns[k]), int(ms[k])
-        dA, dB, dC = h.alloc(n * n * 8), h.alloc(n * m * 8), h.alloc(n * m * 8)
-        h.fill_seeded(n, m, args.seed + k, args.spread, dA, n, dB, m)
+        lda, ldb = n + (n & 1), m + (m & 1)  # even leading dimensions: 16-byte loads
+        dA, dB, dC = h.alloc(n * lda * 8), h.alloc(n * ldb * 8), h.alloc(n * m * 8)
+        h.fill_seeded(n, m, args.seed + k, args.spread, dA, lda, dB, ldb)
         bufs += [dA, dB, dC]
-        probs.append(uvt.Problem(dA.ptr, n, dB.ptr, m, dC.ptr, m, n, m,
+        probs.append(uvt.Problem(dA.ptr, lda, dB.ptr, ldb, dC.ptr, m, n, m,
                                  uvt.PREC_SPLIT if k in split_set else uvt.PREC_F64, 0))
     h.sync()
     t0 = time.perf_counter()
@@ -167,6 +168,89 @@ def batch_leg(args, h, uvt):
             "graph_build_s": round(build_s, 3), "tuning_table": bool(table)}
 
 
+def serve_leg(args, h, uvt):
+    """Served stream: TRMM requests with orders log-uniform in [128, 16384], B
+    columns uniform in {64..4096}, half FP64 and half split class, inputs
+    generated on the device per request, under a 200 GB HBM admission budget.
+    Capacity is measured first (a burst of requests submitted at once); the
+    stream then arrives as a Poisson process at `serve_load` x capacity, and
+    latency is arrival (submit) -> result ready in HBM, over the requests
+    after the warm-up ones."""
+    rng = np.random.default_rng(args.seed + 29)
+    table = os.path.join(ROOT, "uvatrmm_amd", "tune", "f64_tiles.json")
+    table = table if os.path.exists(table) else None
+
+    def draw(count):
+        ns = np.rint(np.exp(rng.uniform(np.log(128), np.log(16384), count))).astype(int)
+        ms = rng.integers(64, 4097, count)
+        precs = np.where(rng.random(count) < 0.5, uvt.PREC_SPLIT, uvt.PREC_F64)
+        return [(int(a), int(b), int(c)) for a, b, c in zip(ns, ms, precs)]
+
+    budget = int(args.serve_budget_gb * 1e9)
+    srv = uvt.Server(h, budget, max_slots=args.serve_slots, slices=args.split_moduli or 13,
+                     table=table)
+    seed = [args.seed * 1000003]
+
+    def submit(req, tag):
+        n, m, prec = req
+        seed[0] += 1
+        return srv.submit_seeded(n, m, prec, seed[0], spread=args.spread, tag=tag)
+
+    try:
+        burst = draw(args.serve_capacity_reqs)
+        for k, r in enumerate(burst):
+            submit(r, k)
+        srv.drain()
+        done = srv.poll(len(burst) + 16)
+        span = max(d["t_done"] for d in done) - min(d["t_submit"] for d in done)
+        capacity = len(burst) / span
+        burst_tflops = sum(float(m) * n * n for n, m, _ in burst) / span * 1e-12
+        rate = args.serve_load * capacity
+        total = args.serve_warmup + args.serve_requests
+        stream = draw(total)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, total))
+        log(f"serve: capacity {capacity:.1f} req/s ({burst_tflops:.2f} TFLOP/s burst); "
+            f"Poisson stream of {total} at {rate:.1f} req/s")
+        t0 = time.perf_counter()
+        for k, r in enumerate(stream):
+            while True:
+                wait = arrivals[k] - (time.perf_counter() - t0)
+                if wait <= 0:
+                    break
+                if wait > 2e-3:
+                    time.sleep(wait - 1e-3)
+            submit(r, 1_000_000 + k)
+            if k % 256 == 0:
+                log(f"serve: submitted {k}/{total}")
+        srv.drain()
+        done = srv.poll(total + 16)
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert len(done) == total and all(d["status"] == uvt.OK for d in done)
+    meas = [d for d in done if d["tag"] - 1_000_000 >= args.serve_warmup]
+    lat = np.sort([d["t_done"] - d["t_submit"] for d in meas])
+    rank = lambda q: float(lat[max(1, int(np.ceil(q * len(lat)))) - 1])
+    span = max(d["t_done"] for d in meas) - min(d["t_submit"] for d in meas)
+    fl = sum(float(stream[d["tag"] - 1_000_000][1]) * stream[d["tag"] - 1_000_000][0] ** 2
+             for d in meas)
+    out = {"requests": len(meas), "warmup_requests": args.serve_warmup,
+           "p50_ms": round(rank(0.50) * 1e3, 4), "p99_ms": round(rank(0.99) * 1e3, 4),
+           "mean_ms": round(float(lat.mean()) * 1e3, 4), "max_ms": round(float(lat[-1]) * 1e3, 4),
+           "throughput_tflops": round(fl / span * 1e-12, 4),
+           "requests_per_s": round(len(meas) / span, 2),
+           "arrival_rate_per_s": round(rate, 2), "load": args.serve_load,
+           "capacity_per_s": round(capacity, 2), "capacity_tflops": round(burst_tflops, 4),
+           "slots": args.serve_slots, "budget_gb": args.serve_budget_gb,
+           "peak_arena_gb": round(st["peak_bytes"] / 1e9, 3),
+           "peak_inflight": st["peak_inflight"],
+           "split_share": round(float(np.mean([p for _, _, p in stream])), 3),
+           "latency": "submit -> result ready in HBM (inputs generated on device per request)"}
+    log(f"serve: p50 {out['p50_ms']} ms p99 {out['p99_ms']} ms, {out['throughput_tflops']} "
+        f"TFLOP/s, {out['requests_per_s']} req/s, peak {st['peak_inflight']} in flight")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,6 +268,13 @@ def main():
     ap.add_argument("--batch", type=int, default=512, help="problems of the batch leg (0 = skip)")
     ap.add_argument("--split-moduli", type=int, default=13,
                     help="moduli of the split-precision leg (0 = skip it)")
+    ap.add_argument("--serve-requests", type=int, default=2000,
+                    help="measured requests of the served stream (0 = skip)")
+    ap.add_argument("--serve-warmup", type=int, default=100)
+    ap.add_argument("--serve-capacity-reqs", type=int, default=300)
+    ap.add_argument("--serve-load", type=float, default=0.7)
+    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the TRMM path shards by columns of B with no "
@@ -227,6 +318,8 @@ def main():
 
     batch = batch_leg(args, h, uvt) if args.batch > 0 else None
 
+    serve = serve_leg(args, h, uvt) if args.serve_requests > 0 else None
+
     cpu = None
     if not args.no_cpu:
         cols = min(m, 16 * args.cpu_cols_per_rank)
@@ -269,6 +362,7 @@ def main():
         "cpu_baseline": cpu,
         "split": split,
         "batch": batch,
+        "serve": serve,
         "norm": norm,
     }
     print(json.dumps(out), flush=True)

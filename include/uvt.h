@@ -137,6 +137,31 @@ uvt_status uvt_trmm_split(uvt_handle h, int64_t n, int64_t m, int32_t slices, co
                           void *dwork, size_t work_bytes, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* BLAS forms of the two per-band calls, device-resident and row-major */
+/* (the CBLAS shim libuvt_cblas maps the host interface onto them).    */
+/* ------------------------------------------------------------------ */
+typedef enum { UVT_NO_TRANS = 0, UVT_TRANS = 1 } uvt_trans;
+typedef enum { UVT_LEFT = 0, UVT_RIGHT = 1 } uvt_side;
+typedef enum { UVT_LOWER = 0, UVT_UPPER = 1 } uvt_uplo;
+typedef enum { UVT_NON_UNIT = 0, UVT_UNIT = 1 } uvt_diag;
+/* C = alpha op(A) op(B) + beta C; C (m x n) is not read when beta == 0, A and
+ * B are not read when k == 0 or alpha == 0.  Replaces the cblas_dgemm call of
+ * UVaTRMM/src/mm_triangular.c:347-360 (RowMajor, NoTrans, NoTrans, alpha 1,
+ * beta 0) and covers the other transposes.  Uses the handle's workspace. */
+uvt_status uvt_dgemm(uvt_handle h, uvt_trans ta, uvt_trans tb, int64_t m, int64_t n, int64_t k,
+                     double alpha, const double *dA, int64_t lda, const double *dB, int64_t ldb,
+                     double beta, double *dC, int64_t ldc, void *stream);
+/* B (m x n) := alpha op(A) B (left, A m x m) or alpha B op(A) (right, A n x n)
+ * in place, A triangular (uplo; only that triangle is read; unit: its
+ * diagonal is taken as 1).  Replaces the cblas_dtrmm call of
+ * mm_triangular.c:374-387 (RowMajor, Left, Lower, NoTrans, NonUnit, alpha 1)
+ * and covers every other side / uplo / trans / diag.  Uses the handle's
+ * workspace. */
+uvt_status uvt_dtrmm(uvt_handle h, uvt_side side, uvt_uplo uplo, uvt_trans ta, uvt_diag diag,
+                     int64_t m, int64_t n, double alpha, const double *dA, int64_t lda,
+                     double *dB, int64_t ldb, void *stream);
+
+/* ------------------------------------------------------------------ */
 /* Batched TRMMs (no counterpart in the reference).                    */
 /* ------------------------------------------------------------------ */
 typedef struct {
@@ -177,6 +202,78 @@ typedef enum { UVT_TILE_128x128 = 0, UVT_TILE_64x64 = 1, UVT_TILE_128x32 = 2 } u
 uvt_status uvt_autotune_f64(uvt_handle h, const char *out_path);
 /* The tile shape the table (NULL: built-in) gives an n x n by n x m problem. */
 uvt_status uvt_tune_choice(const char *tune_table, int64_t n, int64_t m, int32_t *cfg);
+
+/* ------------------------------------------------------------------ */
+/* Serving (no counterpart in the reference).                          */
+/* ------------------------------------------------------------------ */
+/* A server owns a scheduler thread, a device memory arena of hbm_budget
+ * bytes and max_slots streams.  Submitted requests wait in FIFO order and
+ * are admitted as soon as a slot is free and their operands (A, B, C and,
+ * for split requests, the workspace) fit in the unused part of the arena;
+ * admitted requests run concurrently on their slot streams, so requests that
+ * arrive while others are on the GPU join them (continuous batching).  A
+ * request's result does not depend on what else is in flight.  The server
+ * uses only the handle's device (not its workspace), so the handle stays
+ * usable from the caller's thread. */
+typedef struct {
+    size_t hbm_budget; /* bytes of device memory the server may hold */
+    int32_t max_slots; /* requests in flight on the device (1..64) */
+    int32_t slices;    /* moduli for split-class requests */
+    const char *tune_table; /* FP64 tile shapes (NULL: built-in) */
+} uvt_serve_config;
+
+typedef struct {
+    int64_t n, m;
+    int32_t prec;     /* tolerance class: UVT_PREC_F64 (<= 1e-13) or UVT_PREC_SPLIT */
+    int32_t seeded;   /* 1: A and B generated on the device from seed (uvt_fill_seeded);
+                         0: read from the pointers A and B */
+    const double *A;  /* n x n row-major (seeded = 0): host, or device with UVT_REQ_AB_DEVICE */
+    const double *B;  /* n x m row-major (seeded = 0): host, or device with UVT_REQ_AB_DEVICE */
+    double *C;        /* n x m row-major destination: host (copied back before the
+                         completion is posted), device with UVT_REQ_C_DEVICE (written in
+                         place), or NULL to leave the result in the arena and drop it */
+    uint64_t seed;
+    int32_t spread;
+    int32_t flags;    /* UVT_REQ_* */
+    uint64_t tag;     /* returned in the completion */
+} uvt_request;
+
+enum {
+    UVT_REQ_AB_DEVICE = 1, /* A and B are device pointers, read in place (no arena copy) */
+    UVT_REQ_C_DEVICE = 2   /* C is a device pointer, written in place */
+};
+
+typedef struct {
+    uint64_t ticket, tag;
+    uvt_status status;
+    int32_t reserved;
+    double t_submit, t_admit, t_done; /* seconds since the server started */
+} uvt_completion;
+
+typedef struct {
+    int64_t submitted, completed, rejected;
+    double p50_s, p99_s, mean_s, max_s; /* latency: submit -> result ready */
+    double wall_s;                      /* first submit -> last completion */
+    double flops;                       /* sum of m n^2 over completed requests */
+    double peak_bytes;                  /* peak arena use */
+    int32_t peak_inflight;
+    int32_t reserved;
+} uvt_serve_stats;
+
+typedef struct uvt_server_s *uvt_server;
+uvt_status uvt_serve_create(uvt_handle h, const uvt_serve_config *cfg, uvt_server *out);
+/* Non-blocking.  UVT_ERR_BUDGET if the request alone exceeds the budget. */
+uvt_status uvt_serve_submit(uvt_server s, const uvt_request *r, uint64_t *ticket);
+/* Non-blocking: up to cap finished requests (UVT_OK even if none). */
+uvt_status uvt_serve_poll(uvt_server s, uvt_completion *out, int32_t cap, int32_t *count);
+/* Blocks until every submitted request has completed. */
+uvt_status uvt_serve_drain(uvt_server s);
+uvt_status uvt_serve_stats_get(uvt_server s, uvt_serve_stats *st);
+/* Device bytes a request with host operands holds while in flight (A, B, C
+ * and the split workspace); device-resident operands are not charged. */
+uvt_status uvt_serve_request_bytes(const uvt_serve_config *cfg, int64_t n, int64_t m, int32_t prec,
+                                   size_t *bytes);
+uvt_status uvt_serve_destroy(uvt_server s);
 
 /* Row sums of |C| into d_rowsum[n] (device). */
 uvt_status uvt_row_abs_sums(uvt_handle h, int64_t n, int64_t m, const double *dC, int64_t ldc,

@@ -53,3 +53,19 @@ def test_cli_usage_errors_exit_3(uvt):
 
 def test_cli_mapping_abort_codes(uvt):
     assert _cli("7", "balanced", "boxes", "--bands", "7").returncode == 11
+
+
+def test_serve_request_bytes(uvt):
+    """Arena bytes of a request: A, B and C (256-byte aligned pieces), plus the
+    split workspace when the request takes the int8 path (n >= 768, m >= 64)."""
+    a = lambda x: (x + 255) // 256 * 256
+    for n, m in [(100, 7), (768, 63), (2000, 500)]:
+        assert uvt.serve_request_bytes(n, m, uvt.PREC_F64) == a(n * n * 8) + 2 * a(n * m * 8)
+    assert uvt.serve_request_bytes(767, 500, uvt.PREC_SPLIT) == \
+        uvt.serve_request_bytes(767, 500, uvt.PREC_F64)
+    ws = ctypes.c_size_t()
+    assert uvt.lib().uvt_trmm_split_workspace(2000, 500, 13, ctypes.byref(ws)) == uvt.OK
+    assert uvt.serve_request_bytes(2000, 500, uvt.PREC_SPLIT) == \
+        uvt.serve_request_bytes(2000, 500, uvt.PREC_F64) + a(ws.value)
+    with pytest.raises(uvt.UvtError):
+        uvt.serve_request_bytes(0, 5, uvt.PREC_F64)

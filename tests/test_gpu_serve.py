@@ -1,0 +1,145 @@
+"""The TRMM server (uvt_serve_*): requests of mixed sizes and classes admitted
+by HBM budget and free slots.  A request's result does not depend on what
+else is in flight (bit for bit equal to the same request served alone), the
+arena never holds more than the budget, oversize requests are refused, and
+every result agrees with the FP64 oracle within its class's tolerance."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_fro
+
+pytestmark = pytest.mark.gpu
+
+GiB = 1 << 30
+
+
+def say(*a):
+    print("[gpu-serve]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def mixed_requests(count, seed, nmax=3000, mmax=700):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(np.exp(rng.uniform(np.log(48), np.log(nmax))))
+        m = int(np.exp(rng.uniform(np.log(8), np.log(mmax))))
+        prec = 1 if rng.random() < 0.5 else 0
+        out.append((n, m, prec, 5000 + k))
+    return out
+
+
+def serve(uvt, gpu, reqs, budget, slots):
+    srv = uvt.Server(gpu, budget, max_slots=slots, slices=13)
+    outs = [np.full((n, m), np.nan) for n, m, _, _ in reqs]
+    try:
+        for k, (n, m, prec, seed) in enumerate(reqs):
+            srv.submit_seeded(n, m, prec, seed, spread=1, C=outs[k], tag=k)
+            if slots == 1:
+                srv.drain()
+        srv.drain()
+        done = srv.poll()
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert len(done) == len(reqs)
+    assert all(d["status"] == uvt.OK for d in done), [d for d in done if d["status"]]
+    assert sorted(d["tag"] for d in done) == list(range(len(reqs)))
+    for d in done:
+        assert d["t_submit"] <= d["t_admit"] <= d["t_done"]
+    return outs, st
+
+
+def test_serve_alone_equals_mix(uvt, gpu, oracle):
+    reqs = mixed_requests(40, seed=11)
+    say("requests", len(reqs), "alone")
+    alone, st1 = serve(uvt, gpu, reqs, 8 * GiB, 1)
+    assert st1["peak_inflight"] == 1
+    say("mix, 8 slots")
+    mix, st8 = serve(uvt, gpu, reqs, 8 * GiB, 8)
+    say("stats", st8)
+    assert st8["peak_inflight"] > 1
+    assert st8["completed"] == st8["submitted"] == len(reqs)
+    for k, (n, m, prec, seed) in enumerate(reqs):
+        assert np.array_equal(alone[k], mix[k]), (k, n, m, prec)
+    for k, (n, m, prec, seed) in enumerate(reqs[:12]):
+        ref = oracle.trmm(oracle.seeded_A(n, seed, 1), oracle.seeded_B(n, m, seed, 1))
+        err = rel_fro(mix[k], ref)
+        assert err <= (1e-13 if prec == 0 else 1e-12), (k, n, m, prec, err)
+
+
+def test_serve_budget_admission(uvt, gpu):
+    reqs = mixed_requests(30, seed=12, nmax=2500, mmax=600)
+    need = max(uvt.serve_request_bytes(n, m, p) for n, m, p, _ in reqs)
+    budget = int(need * 1.5)
+    say("budget", budget, "largest", need)
+    outs, st = serve(uvt, gpu, reqs, budget, 16)
+    assert st["peak_bytes"] <= budget
+    alone, _ = serve(uvt, gpu, reqs[:6], 8 * GiB, 1)
+    for k in range(6):
+        assert np.array_equal(alone[k], outs[k])
+
+
+def test_serve_rejects_oversize(uvt, gpu):
+    srv = uvt.Server(gpu, 64 << 20, max_slots=2)
+    try:
+        with pytest.raises(uvt.UvtError) as e:
+            srv.submit_seeded(4096, 1024, uvt.PREC_F64, 1)
+        assert e.value.status == uvt.ERR_BUDGET
+        C = np.zeros((256, 100))
+        srv.submit_seeded(256, 100, uvt.PREC_F64, 1, C=C)
+        srv.drain()
+        st = srv.stats()
+        assert st["rejected"] == 1 and st["completed"] == 1
+        assert st["p50_s"] <= st["p99_s"] <= st["max_s"]
+        assert np.isfinite(C).all()
+    finally:
+        srv.close()
+
+
+def test_serve_host_arrays(uvt, gpu, oracle):
+    srv = uvt.Server(gpu, 2 * GiB, max_slots=4)
+    cases = [(300, 77, 0), (1030, 130, 1), (900, 64, 1), (65, 9, 0)]
+    outs = []
+    try:
+        for k, (n, m, prec) in enumerate(cases):
+            A = oracle.seeded_A(n, 70 + k, 1)
+            B = oracle.seeded_B(n, m, 70 + k, 1)
+            C = np.zeros((n, m))
+            srv.submit(A, B, prec, C=C, tag=k)
+            outs.append((A, B, C, prec))
+        srv.drain()
+        assert len(srv.poll()) == len(cases)
+    finally:
+        srv.close()
+    for A, B, C, prec in outs:
+        err = rel_fro(C, oracle.trmm(A, B))
+        assert err <= (1e-13 if prec == 0 else 1e-12), (A.shape, prec, err)
+
+
+def test_serve_device_operands(uvt, gpu, oracle):
+    """Operands already resident in HBM are read and written in place and the
+    arena is charged only for the split workspace."""
+    cases = [(1100, 96, 1), (500, 40, 0), (2048, 256, 0)]
+    srv = uvt.Server(gpu, 1 << 30, max_slots=3)
+    bufs = []
+    try:
+        for k, (n, m, prec) in enumerate(cases):
+            A = oracle.seeded_A(n, 90 + k, 1)
+            B = oracle.seeded_B(n, m, 90 + k, 1)
+            dA, dB, dC = gpu.array(A), gpu.array(B), gpu.alloc(n * m * 8)
+            srv.submit_device(n, m, prec, dA, dB, dC, tag=k)
+            bufs.append((A, B, dC, prec))
+        srv.drain()
+        st = srv.stats()
+    finally:
+        srv.close()
+    ws = max(uvt.serve_request_bytes(n, m, p) - uvt.serve_request_bytes(n, m, uvt.PREC_F64)
+             for n, m, p in cases)
+    assert st["peak_bytes"] <= 3 * ws
+    for A, B, dC, prec in bufs:
+        n, m = B.shape
+        err = rel_fro(dC.download((n, m)), oracle.trmm(A, B))
+        assert err <= (1e-13 if prec == 0 else 1e-12), (n, m, prec, err)

@@ -42,7 +42,28 @@ class Problem(_c.Structure):
                 ("n", i64), ("m", i64), ("prec", i32), ("reserved", i32)]
 
 
+class ServeConfig(_c.Structure):
+    _fields_ = [("hbm_budget", sz), ("max_slots", i32), ("slices", i32), ("tune_table", _c.c_char_p)]
+
+
+class Request(_c.Structure):
+    _fields_ = [("n", i64), ("m", i64), ("prec", i32), ("seeded", i32), ("A", vp), ("B", vp),
+                ("C", vp), ("seed", u64), ("spread", i32), ("flags", i32), ("tag", u64)]
+
+
+class Completion(_c.Structure):
+    _fields_ = [("ticket", u64), ("tag", u64), ("status", i32), ("reserved", i32),
+                ("t_submit", f64), ("t_admit", f64), ("t_done", f64)]
+
+
+class ServeStats(_c.Structure):
+    _fields_ = [("submitted", i64), ("completed", i64), ("rejected", i64), ("p50_s", f64),
+                ("p99_s", f64), ("mean_s", f64), ("max_s", f64), ("wall_s", f64), ("flops", f64),
+                ("peak_bytes", f64), ("peak_inflight", i32), ("reserved", i32)]
+
+
 PREC_F64, PREC_SPLIT = 0, 1
+REQ_AB_DEVICE, REQ_C_DEVICE = 1, 2
 TILE_CFGS = {0: "128x128", 1: "64x64", 2: "128x32"}
 
 
@@ -83,6 +104,13 @@ _SIGS = {
     "uvt_batch_destroy": (i32, [vp]),
     "uvt_autotune_f64": (i32, [vp, _c.c_char_p]),
     "uvt_tune_choice": (i32, [_c.c_char_p, i64, i64, _c.POINTER(i32)]),
+    "uvt_serve_create": (i32, [vp, _c.POINTER(ServeConfig), _c.POINTER(vp)]),
+    "uvt_serve_submit": (i32, [vp, _c.POINTER(Request), _c.POINTER(u64)]),
+    "uvt_serve_poll": (i32, [vp, _c.POINTER(Completion), i32, _c.POINTER(i32)]),
+    "uvt_serve_drain": (i32, [vp]),
+    "uvt_serve_stats_get": (i32, [vp, _c.POINTER(ServeStats)]),
+    "uvt_serve_request_bytes": (i32, [_c.POINTER(ServeConfig), i64, i64, i32, _c.POINTER(sz)]),
+    "uvt_serve_destroy": (i32, [vp]),
     "uvt_row_abs_sums": (i32, [vp, i64, i64, vp, i64, vp, vp]),
     "uvt_inf_norm": (i32, [vp, i64, i64, vp, i64, _c.POINTER(f64), vp]),
     "uvt_fill_closed_form": (i32, [vp, i64, i64, vp, i64, vp, i64, vp]),
@@ -347,6 +375,89 @@ class Batch:
         if self.ptr:
             lib().uvt_batch_destroy(self.ptr)
             self.ptr = None
+
+
+def serve_request_bytes(n, m, prec, slices=13):
+    """Arena bytes one request of this size and class occupies while in flight."""
+    cfg = ServeConfig(1 << 40, 1, slices, None)
+    b = sz()
+    check(lib().uvt_serve_request_bytes(_c.byref(cfg), n, m, prec, _c.byref(b)), "request_bytes")
+    return b.value
+
+
+class Server:
+    """Continuous-batching TRMM server (see uvt_serve_create)."""
+
+    def __init__(self, handle, hbm_budget, max_slots=8, slices=13, table=None):
+        self.h = handle
+        cfg = ServeConfig(hbm_budget, max_slots, slices, table.encode() if table else None)
+        p = vp()
+        check(lib().uvt_serve_create(handle.ptr, _c.byref(cfg), _c.byref(p)), "serve_create")
+        self.ptr = p
+        self._host = {}  # ticket -> host arrays the request reads or writes
+
+    def submit_seeded(self, n, m, prec, seed, spread=1, C=None, tag=0):
+        """Queues a request whose A and B the device generates from seed.
+        C: optional C-contiguous float64 (n, m) array that receives the result."""
+        r = Request(n, m, prec, 1, None, None, None, seed, spread, 0, tag)
+        keep = []
+        if C is not None:
+            assert C.dtype == np.float64 and C.flags.c_contiguous and C.shape == (n, m)
+            r.C = C.ctypes.data
+            keep.append(C)
+        return self._submit(r, keep)
+
+    def submit(self, A, B, prec, C=None, tag=0):
+        """Queues C = tril(A) @ B for host arrays (copied in on admission)."""
+        n, m = B.shape
+        assert A.shape == (n, n) and A.dtype == np.float64 and B.dtype == np.float64
+        assert A.flags.c_contiguous and B.flags.c_contiguous
+        r = Request(n, m, prec, 0, A.ctypes.data, B.ctypes.data, None, 0, 0, 0, tag)
+        keep = [A, B]
+        if C is not None:
+            assert C.dtype == np.float64 and C.flags.c_contiguous and C.shape == (n, m)
+            r.C = C.ctypes.data
+            keep.append(C)
+        return self._submit(r, keep)
+
+    def submit_device(self, n, m, prec, dA, dB, dC, tag=0):
+        """Queues C = tril(A) @ B on device-resident operands (DeviceBuffer;
+        leading dimensions n, m, m), read and written in place.  dA/dB None:
+        The buffers must stay allocated until the request completes."""
+        r = Request(n, m, prec, 0, dA.ptr, dB.ptr, dC.ptr, 0, 0,
+                    REQ_AB_DEVICE | REQ_C_DEVICE, tag)
+        return self._submit(r, [dA, dB, dC])
+
+    def _submit(self, r, keep):
+        t = u64()
+        check(lib().uvt_serve_submit(self.ptr, _c.byref(r), _c.byref(t)), "serve_submit")
+        self._host[t.value] = keep
+        return t.value
+
+    def poll(self, cap=4096):
+        out = (Completion * cap)()
+        k = i32()
+        check(lib().uvt_serve_poll(self.ptr, out, cap, _c.byref(k)), "serve_poll")
+        res = []
+        for c in out[:k.value]:
+            self._host.pop(c.ticket, None)
+            res.append({"ticket": c.ticket, "tag": c.tag, "status": c.status,
+                        "t_submit": c.t_submit, "t_admit": c.t_admit, "t_done": c.t_done})
+        return res
+
+    def drain(self):
+        check(lib().uvt_serve_drain(self.ptr), "serve_drain")
+
+    def stats(self):
+        st = ServeStats()
+        check(lib().uvt_serve_stats_get(self.ptr, _c.byref(st)), "serve_stats")
+        return {k: getattr(st, k) for k, _ in ServeStats._fields_ if k != "reserved"}
+
+    def close(self):
+        if self.ptr:
+            lib().uvt_serve_destroy(self.ptr)
+            self.ptr = None
+            self._host.clear()
 
 
 def autotune(handle, path):

@@ -95,7 +95,7 @@ int main(int argc, char **argv)
     else if (!strcmp(argv[3], "trapezoid")) shape = UVT_SHAPE_TRAPEZOID;
     else show_usage(argv[0], "Unknown name of communication shape");
 
-    long bands = 1, m = -1, reps = 1, seed = 1, slices = 7;
+    long bands = 1, m = -1, reps = 1, seed = 1, slices = 13;
     int device = 0, json = 0, seeded = 0, spread = 0, split = 0;
     const char *dump = NULL;
     for (int i = 4; i < argc; i++) {

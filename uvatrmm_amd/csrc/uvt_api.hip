@@ -29,6 +29,24 @@ void k_trmm_tiles_f64(const uvt_tile* __restrict__ tiles, const double* __restri
                                       C, ldc, m, smem);
 }
 
+// The trapezoid walk without a tile list: workgroup b takes row tile
+// nrt-1-b/nct (longest reduction first) and column tile b%nct.
+template <int TM, int TN, bool VEC>
+__global__ __launch_bounds__((64 * TileWaves<TM, TN>::M * TileWaves<TM, TN>::N),
+                             (TileWaves<TM, TN>::WAVES_PER_EU))
+void k_trmm_trapezoid_f64(int64_t n, int64_t m, int nrt, int nct, const double* __restrict__ A,
+                          int64_t lda, const double* __restrict__ B, int64_t ldb,
+                          double* __restrict__ C, int64_t ldc)
+{
+    using W = TileWaves<TM, TN>;
+    __shared__ double smem[F64Cfg<TM, TN, W::M, W::N>::SMEM_DOUBLES];
+    const int rt = nrt - 1 - (int)(blockIdx.x / nct), ct = (int)(blockIdx.x % nct);
+    const int r0 = rt * TM;
+    const int rows = (int)(n - r0 < TM ? n - r0 : TM);
+    f64_tile<TM, TN, VEC, W::M, W::N>(r0, rows, ct * TN, 0, r0 + rows, kTileLower, A, lda, B, ldb, C,
+                                      ldc, m, smem);
+}
+
 // One wave per row: sum_j |C[i][j]|, lanes strided over columns, then a
 // fixed butterfly, so the order (and the result) is the same on every run.
 __global__ __launch_bounds__(256) void k_row_abs_sums(int64_t n, int64_t m,
@@ -238,6 +256,37 @@ uvt_status uvt_launch_tiles_f64(int32_t cfg, const uvt_tile* dtiles, int64_t nti
         launch_tiles<128, 32>(dtiles, ntiles, vec, A, lda, B, ldb, C, ldc, m, s);
     else
         launch_tiles<128, 128>(dtiles, ntiles, vec, A, lda, B, ldb, C, ldc, m, s);
+    UVT_HIP(hipGetLastError());
+    return UVT_OK;
+}
+
+template <int TM, int TN>
+static void launch_trapezoid(bool vec, int64_t n, int64_t m, const double* A, int64_t lda,
+                             const double* B, int64_t ldb, double* C, int64_t ldc, hipStream_t s)
+{
+    const int nrt = (int)((n + TM - 1) / TM), nct = (int)((m + TN - 1) / TN);
+    const dim3 grid((unsigned)((int64_t)nrt * nct)), block(64 * TileWaves<TM, TN>::M * TileWaves<TM, TN>::N);
+    if (vec)
+        hipLaunchKernelGGL((k_trmm_trapezoid_f64<TM, TN, true>), grid, block, 0, s, n, m, nrt, nct, A,
+                           lda, B, ldb, C, ldc);
+    else
+        hipLaunchKernelGGL((k_trmm_trapezoid_f64<TM, TN, false>), grid, block, 0, s, n, m, nrt, nct,
+                           A, lda, B, ldb, C, ldc);
+}
+
+uvt_status uvt_launch_trapezoid_f64(int32_t cfg, int64_t n, int64_t m, const double* A, int64_t lda,
+                                    const double* B, int64_t ldb, double* C, int64_t ldc,
+                                    hipStream_t s)
+{
+    if (n <= 0 || m <= 0 || lda < n || ldb < m || ldc < m) return UVT_ERR_ARG;
+    if (n > 0x3FFFFFFF || m > 0x3FFFFFFF) return UVT_ERR_SIZE;
+    const int64_t tm = uvt_cfg_tile_m(cfg), tn = uvt_cfg_tile_n(cfg);
+    if (((n + tm - 1) / tm) * ((m + tn - 1) / tn) > 0x7FFFFFFF) return UVT_ERR_SIZE;
+    const bool vec = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && lda % 2 == 0 &&
+                     ldb % 2 == 0;
+    if (cfg == UVT_CFG_64x64) launch_trapezoid<64, 64>(vec, n, m, A, lda, B, ldb, C, ldc, s);
+    else if (cfg == UVT_CFG_128x32) launch_trapezoid<128, 32>(vec, n, m, A, lda, B, ldb, C, ldc, s);
+    else launch_trapezoid<128, 128>(vec, n, m, A, lda, B, ldb, C, ldc, s);
     UVT_HIP(hipGetLastError());
     return UVT_OK;
 }

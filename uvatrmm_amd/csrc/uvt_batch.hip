@@ -3,14 +3,14 @@
 // in the reference, whose ranks each run one N x N product.
 //
 // A batch is captured once into a graph:
-//   main branch:  zero the work counters (a fill kernel, not a memset node)
-//                 -> one persistent kernel per FP64 tile shape, each draining
-//                    a longest-first queue of tiles gathered from every FP64
-//                    problem of that shape
-//   side branch:  the integer stages of the split problems (scales,
-//                 residues, int8 products)
-//   join -> the split reconstruction, which adds the integer box part to the
-//           diagonal blocks the FP64 queue stored.
+//   zero the work counters (a fill kernel, not a memset node)
+//   -> scales and residues of the split problems
+//   -> fork: main branch: one persistent kernel per FP64 tile shape, each
+//                         draining a longest-first queue of tiles gathered
+//                         from every problem of that shape
+//            side branch: the int8 residue products of the split problems
+//   -> join -> the split reconstruction, which adds the integer box part to
+//              the diagonal blocks the FP64 queue stored.
 // The FP64 queue holds every tile of the FP64 problems and the diagonal-block
 // tiles of the split problems.  Split-class problems too small for the
 // integer box to pay off (n < 768 or m < 64) run wholly in FP64, which meets
@@ -125,11 +125,6 @@ void launch_persistent(int cfg, bool vec, int grid, const BatchTile* tiles, int 
 
 // ---- tuning table ------------------------------------------------------------
 
-struct TuneEntry {
-    int64_t n_max, m_max;
-    int cfg;
-};
-
 const char* cfg_name(int cfg)
 {
     return cfg == UVT_CFG_64x64 ? "64x64" : cfg == UVT_CFG_128x32 ? "128x32" : "128x128";
@@ -143,10 +138,12 @@ int cfg_from_name(const std::string& s)
     return -1;
 }
 
+}  // namespace
+
 // Reads the table written by uvt_autotune_f64: a JSON object whose "table"
 // array holds {"n_max": N, "m_max": M, "cfg": "AxB", ...} entries.  Only
 // those three keys are read; entries are matched in file order.
-bool load_table(const char* path, std::vector<TuneEntry>& out)
+bool uvt_tune_load(const char* path, std::vector<uvt_tune_entry>& out)
 {
     FILE* f = fopen(path, "rb");
     if (!f) return false;
@@ -172,7 +169,7 @@ bool load_table(const char* path, std::vector<TuneEntry>& out)
             *v = strtoll(e.c_str() + c + 1, nullptr, 10);
             return true;
         };
-        TuneEntry t;
+        uvt_tune_entry t;
         const size_t k = e.find("\"cfg\"");
         if (!num("\"n_max\"", &t.n_max) || !num("\"m_max\"", &t.m_max) || k == std::string::npos)
             return false;
@@ -187,12 +184,14 @@ bool load_table(const char* path, std::vector<TuneEntry>& out)
     return !out.empty();
 }
 
-int choose(const std::vector<TuneEntry>& table, int64_t n, int64_t m)
+int uvt_tune_pick(const std::vector<uvt_tune_entry>& table, int64_t n, int64_t m)
 {
-    for (const TuneEntry& t : table)
+    for (const uvt_tune_entry& t : table)
         if (n <= t.n_max && m <= t.m_max) return t.cfg;
     return uvt_default_cfg(n, m);
 }
+
+namespace {
 
 // Tiles of the lower trapezoid of one problem.  diag_only: only the 256-row
 // diagonal blocks (k from the start of the row's 256-row tile), the FP64 part
@@ -319,8 +318,8 @@ extern "C" uvt_status uvt_batch_create(uvt_handle h, const uvt_problem* probs, i
         uvt_status st = check(probs[i]);
         if (st != UVT_OK) return st;
     }
-    std::vector<TuneEntry> table;
-    if (tune_table && !load_table(tune_table, table)) return UVT_ERR_ARG;
+    std::vector<uvt_tune_entry> table;
+    if (tune_table && !uvt_tune_load(tune_table, table)) return UVT_ERR_ARG;
     UVT_HIP(hipSetDevice(h->device));
     uvt_batch b = new (std::nothrow) uvt_batch_s;
     if (!b) return UVT_ERR_NOMEM;
@@ -338,7 +337,7 @@ extern "C" uvt_status uvt_batch_create(uvt_handle h, const uvt_problem* probs, i
         const int slot = (int)b->f64_probs.size();
         b->f64_index.push_back(slot);
         b->f64_probs.push_back(f64_desc(p));
-        const int cfg = table.empty() ? uvt_default_cfg(p.n, p.m) : choose(table, p.n, p.m);
+        const int cfg = table.empty() ? uvt_default_cfg(p.n, p.m) : uvt_tune_pick(table, p.n, p.m);
         const bool use_int8 = p.prec == UVT_PREC_SPLIT && split_pays_off(p.n, p.m);
         trapezoid_tiles(slot, p.n, p.m, cfg, b->tiles[cfg * 2 + b->f64_probs.back().vec], use_int8);
         b->int8.push_back(use_int8);
@@ -402,17 +401,24 @@ extern "C" uvt_status uvt_batch_create(uvt_handle h, const uvt_problem* probs, i
     BATCH_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
     BATCH_HIP(hipEventCreateWithFlags(&b->join, hipEventDisableTiming));
     BATCH_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(128), 0, b->cap, b->d_counters, 128);
+    // scales and residues of the split problems first: short kernels that
+    // would be starved behind the persistent FP64 workgroups
+    if (!b->sp.empty())
+        st = uvt_split_batch_launch(b->sp.data(), (int)b->sp.size(), slices, b->d_split_work,
+                                    b->d_split_desc, b->d_split_lut, b->split_totals, b->split_lut_off,
+                                    b->cap, UVT_SPLIT_PREP);
     BATCH_HIP(hipEventRecord(b->fork, b->cap));
     BATCH_HIP(hipStreamWaitEvent(b->side, b->fork, 0));
-    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(128), 0, b->cap, b->d_counters, 128);
+    // int8 residue products on the side branch, the FP64 queue on the main one
+    if (st == UVT_OK && !b->sp.empty())
+        st = uvt_split_batch_launch(b->sp.data(), (int)b->sp.size(), slices, b->d_split_work,
+                                    b->d_split_desc, b->d_split_lut, b->split_totals, b->split_lut_off,
+                                    b->side, UVT_SPLIT_GEMM);
     for (int q = 0; q < 2 * UVT_NUM_CFGS; ++q)
         if (!b->tiles[q].empty())
             launch_persistent(q / 2, q & 1, grids[q / 2], b->d_tiles[q], (int)b->tiles[q].size(),
                               b->d_probs, b->d_counters + 16 * q, b->cap);
-    if (!b->sp.empty())
-        st = uvt_split_batch_launch(b->sp.data(), (int)b->sp.size(), slices, b->d_split_work,
-                                    b->d_split_desc, b->d_split_lut, b->split_totals, b->split_lut_off,
-                                    b->side, UVT_SPLIT_BOX);
     BATCH_HIP(hipEventRecord(b->join, b->side));
     BATCH_HIP(hipStreamWaitEvent(b->cap, b->join, 0));
     // the reconstruction adds to the diagonal blocks the FP64 queue stored
@@ -495,9 +501,9 @@ extern "C" uvt_status uvt_tune_choice(const char* tune_table, int64_t n, int64_t
         *cfg = uvt_default_cfg(n, m);
         return UVT_OK;
     }
-    std::vector<TuneEntry> table;
-    if (!load_table(tune_table, table)) return UVT_ERR_ARG;
-    *cfg = choose(table, n, m);
+    std::vector<uvt_tune_entry> table;
+    if (!uvt_tune_load(tune_table, table)) return UVT_ERR_ARG;
+    *cfg = uvt_tune_pick(table, n, m);
     return UVT_OK;
 }
 

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 
+#include <vector>
+
 #include "uvt_internal.h"
 
 struct uvt_handle_s {
@@ -30,3 +32,23 @@ uvt_status uvt_handle_workspace(uvt_handle h, size_t bytes, void** p);
 uvt_status uvt_launch_tiles_f64(int32_t cfg, const uvt_tile* dtiles, int64_t ntiles,
                                 const double* A, int64_t lda, const double* B, int64_t ldb,
                                 double* C, int64_t ldc, int64_t m, hipStream_t s);
+// FP64 C = tril(A) * B as one trapezoid walk with the given tile shape,
+// tile coordinates from the workgroup index (no plan, nothing to upload).
+uvt_status uvt_launch_trapezoid_f64(int32_t cfg, int64_t n, int64_t m, const double* A, int64_t lda,
+                                    const double* B, int64_t ldb, double* C, int64_t ldc,
+                                    hipStream_t s);
+// FP64 C[0:M, 0:N] = A[0:M, 0:K] * B[0:K, 0:N] (no triangle), tile coordinates
+// from the workgroup index.
+uvt_status uvt_launch_gemm_f64(int32_t cfg, int64_t M, int64_t N, int64_t K, const double* A,
+                               int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc,
+                               hipStream_t s);
+
+// Tile-shape table written by uvt_autotune_f64: entries matched in order,
+// the first with n <= n_max and m <= m_max gives the shape.
+struct uvt_tune_entry {
+    int64_t n_max, m_max;
+    int cfg;
+};
+bool uvt_tune_load(const char* path, std::vector<uvt_tune_entry>& out);
+// Shape for (n, m); the built-in default when no entry matches.
+int uvt_tune_pick(const std::vector<uvt_tune_entry>& table, int64_t n, int64_t m);

@@ -1,0 +1,440 @@
+// uvt_serve.hip -- a TRMM server: requests of mixed sizes and tolerance
+// classes arrive over time, are admitted by HBM budget and free slots, run
+// concurrently on per-slot streams, and are timed from arrival to result.
+// No counterpart in the reference (host code only; the kernels are those of
+// uvt_api.hip and uvt_split.hip).
+//
+// One scheduler thread owns every HIP call of the server:
+//   loop: collect finished slots (event query) -> free their arena blocks ->
+//         admit pending requests in FIFO order while a slot is free and the
+//         head request's operands fit in the arena -> launch them (inputs,
+//         product, optional copy-back) on the slot's stream -> sleep until a
+//         submission or ~50 us.
+// A request's kernels depend only on its own (n, m, class), so its result is
+// the same alone and in the mix.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "uvt_internal.h"
+#include "uvt_runtime.hpp"
+
+namespace {
+
+size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// First-fit allocator over one device allocation, 256-byte granules.
+class Arena {
+public:
+    bool init(size_t bytes)
+    {
+        size_ = align_up(bytes);
+        if (hipMalloc(&base_, size_) != hipSuccess) {
+            (void)hipGetLastError();
+            base_ = nullptr;
+            return false;
+        }
+        free_[0] = size_;
+        return true;
+    }
+    void release()
+    {
+        if (base_) (void)hipFree(base_);
+        base_ = nullptr;
+        free_.clear();
+    }
+    // offset of a block of `bytes`, or SIZE_MAX
+    size_t alloc(size_t bytes)
+    {
+        bytes = align_up(bytes);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                const size_t off = it->first, rest = it->second - bytes;
+                free_.erase(it);
+                if (rest) free_[off + bytes] = rest;
+                used_ += bytes;
+                return off;
+            }
+        }
+        return SIZE_MAX;
+    }
+    void free(size_t off, size_t bytes)
+    {
+        bytes = align_up(bytes);
+        used_ -= bytes;
+        auto next = free_.lower_bound(off);
+        if (next != free_.end() && off + bytes == next->first) {
+            bytes += next->second;
+            next = free_.erase(next);
+        }
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                prev->second += bytes;
+                return;
+            }
+        }
+        free_[off] = bytes;
+    }
+    char* at(size_t off) const { return (char*)base_ + off; }
+    size_t used() const { return used_; }
+    size_t size() const { return size_; }
+
+private:
+    void* base_ = nullptr;
+    size_t size_ = 0, used_ = 0;
+    std::map<size_t, size_t> free_;
+};
+
+struct Pending {
+    uint64_t ticket;
+    uvt_request r;
+    double t_submit;
+    size_t bytes;
+};
+
+struct Block {
+    size_t off, bytes;
+};
+
+struct Slot {
+    bool busy = false;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    Pending p{};
+    double t_admit = 0.0;
+    double* dC = nullptr;
+    std::vector<Block> blocks;
+    uvt_status status = UVT_OK;
+};
+
+bool split_pays_off(int64_t n, int64_t m) { return n >= 768 && m >= 64; }
+
+}  // namespace
+
+struct uvt_server_s {
+    uvt_handle h = nullptr;
+    uvt_serve_config cfg{};
+    std::vector<uvt_tune_entry> tune;  // empty: built-in tile-shape choice
+    std::thread th;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_drain;
+    std::deque<Pending> pending;
+    std::vector<uvt_completion> done;
+    std::vector<Slot> slots;
+    Arena arena;
+    bool stop = false;
+    uint64_t next_ticket = 1;
+    int inflight = 0;
+    // statistics
+    int64_t submitted = 0, completed = 0, rejected = 0;
+    std::vector<double> latencies;
+    double first_submit = -1.0, last_done = 0.0, flops = 0.0;
+    size_t peak_bytes = 0;
+    int peak_inflight = 0;
+    std::chrono::steady_clock::time_point t0;
+
+    double now() const
+    {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+static size_t request_bytes(const uvt_serve_config& cfg, int64_t n, int64_t m, int32_t prec,
+                            bool ab_in_arena, bool c_in_arena)
+{
+    size_t b = 0;
+    if (ab_in_arena) b += align_up((size_t)n * n * 8) + align_up((size_t)n * m * 8);
+    if (c_in_arena) b += align_up((size_t)n * m * 8);
+    if (prec == UVT_PREC_SPLIT && split_pays_off(n, m)) {
+        size_t w = 0;
+        if (uvt_trmm_split_workspace(n, m, cfg.slices, &w) == UVT_OK) b += align_up(w);
+    }
+    return b;
+}
+
+static bool ab_in_arena(const uvt_request& r) { return r.seeded || !(r.flags & UVT_REQ_AB_DEVICE); }
+static bool c_in_arena(const uvt_request& r) { return !(r.C && (r.flags & UVT_REQ_C_DEVICE)); }
+
+extern "C" uvt_status uvt_serve_request_bytes(const uvt_serve_config* cfg, int64_t n, int64_t m,
+                                              int32_t prec, size_t* bytes)
+{
+    if (!cfg || !bytes || n <= 0 || m <= 0) return UVT_ERR_ARG;
+    *bytes = request_bytes(*cfg, n, m, prec, true, true);
+    return UVT_OK;
+}
+
+// Enqueues one admitted request on its slot (scheduler thread, lock not held).
+static uvt_status launch(uvt_server s, Slot& sl)
+{
+    const uvt_request& r = sl.p.r;
+    const int64_t n = r.n, m = r.m;
+    auto block = [&](size_t bytes) -> char* {
+        const size_t off = s->arena.alloc(bytes);
+        if (off == SIZE_MAX) return nullptr;
+        sl.blocks.push_back(Block{off, bytes});
+        return s->arena.at(off);
+    };
+    // admission reserved the total; the blocks below come out of it
+    const bool own_ab = ab_in_arena(r);
+    const double* dA = own_ab ? (const double*)block((size_t)n * n * 8) : r.A;
+    const double* dB = own_ab ? (const double*)block((size_t)n * m * 8) : r.B;
+    double* dC = c_in_arena(r) ? (double*)block((size_t)n * m * 8) : r.C;
+    const bool int8 = r.prec == UVT_PREC_SPLIT && split_pays_off(n, m);
+    size_t wbytes = 0;
+    void* work = nullptr;
+    if (int8) {
+        if (uvt_trmm_split_workspace(n, m, s->cfg.slices, &wbytes) != UVT_OK) return UVT_ERR_ARG;
+        work = block(wbytes);
+    }
+    if (!dA || !dB || !dC || (int8 && !work)) return UVT_ERR_NOMEM;
+    sl.dC = dC;
+    hipStream_t st = sl.stream;
+    uvt_status e = UVT_OK;
+    if (r.seeded) {
+        e = uvt_fill_seeded(s->h, n, m, r.seed, r.spread, (double*)dA, n, (double*)dB, m, st);
+    } else if (own_ab) {
+        UVT_HIP(hipMemcpyAsync((void*)dA, r.A, (size_t)n * n * 8, hipMemcpyHostToDevice, st));
+        UVT_HIP(hipMemcpyAsync((void*)dB, r.B, (size_t)n * m * 8, hipMemcpyHostToDevice, st));
+    }
+    if (e != UVT_OK) return e;
+    if (int8) {
+        e = uvt_trmm_split(s->h, n, m, s->cfg.slices, dA, n, dB, m, dC, m, work, wbytes, st);
+    } else {
+        const int tile = s->tune.empty() ? uvt_default_cfg(n, m) : uvt_tune_pick(s->tune, n, m);
+        e = uvt_launch_trapezoid_f64(tile, n, m, dA, n, dB, m, dC, m, st);
+    }
+    if (e != UVT_OK) return e;
+    UVT_HIP(hipEventRecord(sl.done, st));
+    return UVT_OK;
+}
+
+static void scheduler(uvt_server s)
+{
+    (void)hipSetDevice(s->h->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        bool progress = false;
+        // 1. finished slots
+        for (Slot& sl : s->slots) {
+            if (!sl.busy) continue;
+            const hipError_t q = sl.status == UVT_OK ? hipEventQuery(sl.done) : hipSuccess;
+            if (q == hipErrorNotReady) continue;
+            if (q != hipSuccess) {
+                (void)hipGetLastError();
+                sl.status = UVT_ERR_HIP;
+            }
+            if (sl.status == UVT_OK && sl.p.r.C && c_in_arena(sl.p.r)) {
+                lk.unlock();
+                const hipError_t c = hipMemcpy(sl.p.r.C, sl.dC, (size_t)sl.p.r.n * sl.p.r.m * 8,
+                                               hipMemcpyDeviceToHost);
+                lk.lock();
+                if (c != hipSuccess) {
+                    (void)hipGetLastError();
+                    sl.status = UVT_ERR_HIP;
+                }
+            }
+            const double t = s->now();
+            for (const Block& b : sl.blocks) s->arena.free(b.off, b.bytes);
+            sl.blocks.clear();
+            uvt_completion c;
+            memset(&c, 0, sizeof c);
+            c.ticket = sl.p.ticket;
+            c.tag = sl.p.r.tag;
+            c.status = sl.status;
+            c.t_submit = sl.p.t_submit;
+            c.t_admit = sl.t_admit;
+            c.t_done = t;
+            s->done.push_back(c);
+            s->completed++;
+            if (sl.status == UVT_OK) {
+                s->latencies.push_back(t - sl.p.t_submit);
+                s->flops += (double)sl.p.r.m * (double)sl.p.r.n * (double)sl.p.r.n;
+            }
+            s->last_done = std::max(s->last_done, t);
+            sl.busy = false;
+            s->inflight--;
+            progress = true;
+        }
+        if (progress) s->cv_drain.notify_all();
+        // 2. admission, FIFO
+        while (!s->pending.empty()) {
+            Slot* free_slot = nullptr;
+            for (Slot& sl : s->slots)
+                if (!sl.busy) {
+                    free_slot = &sl;
+                    break;
+                }
+            if (!free_slot) break;
+            const Pending& head = s->pending.front();
+            if (s->arena.used() + head.bytes > s->arena.size()) break;  // wait for memory
+            free_slot->busy = true;
+            free_slot->p = head;
+            free_slot->t_admit = s->now();
+            free_slot->status = UVT_OK;
+            s->pending.pop_front();
+            s->inflight++;
+            s->peak_inflight = std::max(s->peak_inflight, s->inflight);
+            // the arena and the slot belong to this thread: launch unlocked, so
+            // submit and poll never wait behind copies or kernel launches
+            lk.unlock();
+            const uvt_status ls = launch(s, *free_slot);
+            // a failed launch may have enqueued part of its work: let it finish
+            // before the blocks go back to the arena
+            if (ls != UVT_OK) (void)hipStreamSynchronize(free_slot->stream);
+            lk.lock();
+            free_slot->status = ls;
+            s->peak_bytes = std::max(s->peak_bytes, s->arena.used());
+            progress = true;
+        }
+        if (s->stop && s->pending.empty() && s->inflight == 0) break;
+        if (!progress) {
+            if (s->inflight > 0) s->cv_work.wait_for(lk, std::chrono::microseconds(50));
+            else s->cv_work.wait(lk);
+        }
+    }
+}
+
+extern "C" uvt_status uvt_serve_create(uvt_handle h, const uvt_serve_config* cfg, uvt_server* out)
+{
+    if (!h || !cfg || !out || cfg->max_slots < 1 || cfg->max_slots > 64 || cfg->slices < 2 ||
+        cfg->slices > 14 || cfg->hbm_budget < 4096)
+        return UVT_ERR_ARG;
+    std::vector<uvt_tune_entry> tune;
+    if (cfg->tune_table && !uvt_tune_load(cfg->tune_table, tune)) return UVT_ERR_ARG;
+    UVT_HIP(hipSetDevice(h->device));
+    uvt_server s = new (std::nothrow) uvt_server_s;
+    if (!s) return UVT_ERR_NOMEM;
+    s->h = h;
+    s->cfg = *cfg;
+    s->cfg.tune_table = nullptr;  // the caller's string need not outlive create
+    s->tune = tune;
+    s->t0 = std::chrono::steady_clock::now();
+    if (!s->arena.init(cfg->hbm_budget)) {
+        delete s;
+        return UVT_ERR_NOMEM;
+    }
+    s->slots.resize(cfg->max_slots);
+    for (Slot& sl : s->slots) {
+        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            for (Slot& x : s->slots) {
+                if (x.stream) (void)hipStreamDestroy(x.stream);
+                if (x.done) (void)hipEventDestroy(x.done);
+            }
+            s->arena.release();
+            delete s;
+            return UVT_ERR_HIP;
+        }
+    }
+    s->th = std::thread(scheduler, s);
+    *out = s;
+    return UVT_OK;
+}
+
+extern "C" uvt_status uvt_serve_submit(uvt_server s, const uvt_request* r, uint64_t* ticket)
+{
+    if (!s || !r || r->n <= 0 || r->m <= 0) return UVT_ERR_ARG;
+    if (r->prec != UVT_PREC_F64 && r->prec != UVT_PREC_SPLIT) return UVT_ERR_ARG;
+    if (!r->seeded && (!r->A || !r->B)) return UVT_ERR_ARG;
+    if (r->flags & ~(UVT_REQ_AB_DEVICE | UVT_REQ_C_DEVICE)) return UVT_ERR_ARG;
+    if ((r->flags & UVT_REQ_C_DEVICE) && !r->C) return UVT_ERR_ARG;
+    if (r->n > 65536 || r->m > 0x3FFFFFFF) return UVT_ERR_SIZE;
+    const size_t bytes = request_bytes(s->cfg, r->n, r->m, r->prec, ab_in_arena(*r), c_in_arena(*r));
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return UVT_ERR_STATE;
+    if (bytes > s->arena.size()) {
+        s->rejected++;
+        return UVT_ERR_BUDGET;
+    }
+    Pending p;
+    p.ticket = s->next_ticket++;
+    p.r = *r;
+    p.t_submit = s->now();
+    p.bytes = bytes;
+    if (s->first_submit < 0) s->first_submit = p.t_submit;
+    s->pending.push_back(p);
+    s->submitted++;
+    if (ticket) *ticket = p.ticket;
+    s->cv_work.notify_one();
+    return UVT_OK;
+}
+
+extern "C" uvt_status uvt_serve_poll(uvt_server s, uvt_completion* out, int32_t cap, int32_t* count)
+{
+    if (!s || !count || (cap > 0 && !out)) return UVT_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    const int32_t k = (int32_t)std::min<size_t>(cap > 0 ? cap : 0, s->done.size());
+    for (int32_t i = 0; i < k; ++i) out[i] = s->done[i];
+    s->done.erase(s->done.begin(), s->done.begin() + k);
+    *count = k;
+    return UVT_OK;
+}
+
+extern "C" uvt_status uvt_serve_drain(uvt_server s)
+{
+    if (!s) return UVT_ERR_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_drain.wait(lk, [&] { return s->pending.empty() && s->inflight == 0; });
+    return UVT_OK;
+}
+
+extern "C" uvt_status uvt_serve_stats_get(uvt_server s, uvt_serve_stats* st)
+{
+    if (!s || !st) return UVT_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    memset(st, 0, sizeof *st);
+    st->submitted = s->submitted;
+    st->completed = s->completed;
+    st->rejected = s->rejected;
+    std::vector<double> v = s->latencies;
+    if (!v.empty()) {
+        std::sort(v.begin(), v.end());
+        auto rank = [&](double q) {  // nearest-rank percentile
+            size_t k = (size_t)std::ceil(q * (double)v.size());
+            return v[std::min(v.size(), std::max<size_t>(k, 1)) - 1];
+        };
+        st->p50_s = rank(0.50);
+        st->p99_s = rank(0.99);
+        double sum = 0.0;
+        for (double x : v) sum += x;
+        st->mean_s = sum / (double)v.size();
+        st->max_s = v.back();
+    }
+    st->wall_s = s->first_submit >= 0 ? s->last_done - s->first_submit : 0.0;
+    st->flops = s->flops;
+    st->peak_bytes = (double)s->peak_bytes;
+    st->peak_inflight = s->peak_inflight;
+    return UVT_OK;
+}
+
+extern "C" uvt_status uvt_serve_destroy(uvt_server s)
+{
+    if (!s) return UVT_ERR_ARG;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->th.joinable()) s->th.join();
+    (void)hipSetDevice(s->h->device);
+    for (Slot& sl : s->slots) {
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+    }
+    s->arena.release();
+    delete s;
+    return UVT_OK;
+}

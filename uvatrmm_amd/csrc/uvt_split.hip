@@ -687,22 +687,22 @@ uvt_status uvt_split_batch_launch(const uvt_split_problem* single, int count, in
     if ((stages & UVT_SPLIT_DIAG) && totals[K_DIAG])
         hipLaunchKernelGGL(k_diag_f64, dim3((unsigned)totals[K_DIAG]), dim3(512), 0, s, one, table,
                            lut(K_DIAG), count);
-    if ((stages & UVT_SPLIT_BOX) && totals[K_ALPHA])
+    if ((stages & UVT_SPLIT_PREP) && totals[K_ALPHA])
         hipLaunchKernelGGL(k_alpha, dim3((unsigned)totals[K_ALPHA]), dim3(256), 0, s, one, table,
                            lut(K_ALPHA), count);
-    if ((stages & UVT_SPLIT_BOX) && totals[K_BPART])
+    if ((stages & UVT_SPLIT_PREP) && totals[K_BPART])
         hipLaunchKernelGGL(k_beta_part, dim3((unsigned)totals[K_BPART]), dim3(256), 0, s, one, table,
                            lut(K_BPART), count);
-    if ((stages & UVT_SPLIT_BOX) && totals[K_BETA])
+    if ((stages & UVT_SPLIT_PREP) && totals[K_BETA])
         hipLaunchKernelGGL(k_beta, dim3((unsigned)totals[K_BETA]), dim3(256), 0, s, one, table,
                            lut(K_BETA), count);
-    if ((stages & UVT_SPLIT_BOX) && totals[K_RA])
+    if ((stages & UVT_SPLIT_PREP) && totals[K_RA])
         hipLaunchKernelGGL(k_resid_a, dim3((unsigned)totals[K_RA]), dim3(256), 0, s, one, table,
                            lut(K_RA), count, mods);
-    if ((stages & UVT_SPLIT_BOX) && totals[K_RB])
+    if ((stages & UVT_SPLIT_PREP) && totals[K_RB])
         hipLaunchKernelGGL(k_resid_b, dim3((unsigned)totals[K_RB]), dim3(256), 0, s, one, table,
                            lut(K_RB), count, mods);
-    if ((stages & UVT_SPLIT_BOX) && totals[K_GEMM])
+    if ((stages & UVT_SPLIT_GEMM) && totals[K_GEMM])
         hipLaunchKernelGGL(k_gemm_i8, dim3((unsigned)totals[K_GEMM]), dim3(512), 0, s, one, table,
                            lut(K_GEMM), count, mods);
     if ((stages & UVT_SPLIT_REC) && totals[K_REC])

@@ -29,10 +29,16 @@ uvt_status uvt_split_batch_sizes(const uvt_split_problem* probs, int count, int 
 uvt_status uvt_split_batch_build(const uvt_split_problem* probs, int count, int nmod, void* dwork,
                                  std::vector<char>& desc_img, std::vector<int>& lut_img,
                                  int64_t totals[8], int64_t lut_off[8]);
-// Stage groups: the FP64 diagonal blocks (store C), the integer box part up
-// to the residue products, and the reconstruction (adds to C, so it must run
-// after the diagonal blocks).
-enum { UVT_SPLIT_DIAG = 1, UVT_SPLIT_BOX = 2, UVT_SPLIT_REC = 4, UVT_SPLIT_ALL = 7 };
+// Stage groups.  The diagonal blocks store C; the reconstruction adds to C,
+// so it must run after them.
+enum {
+    UVT_SPLIT_DIAG = 1,  // FP64 diagonal blocks
+    UVT_SPLIT_PREP = 2,  // scales and residues
+    UVT_SPLIT_GEMM = 4,  // int8 residue products
+    UVT_SPLIT_REC = 8,   // reconstruction
+    UVT_SPLIT_BOX = UVT_SPLIT_PREP | UVT_SPLIT_GEMM,
+    UVT_SPLIT_ALL = 15
+};
 // Enqueues the selected stages of the batch on stream s.
 uvt_status uvt_split_batch_launch(const uvt_split_problem* single, int count, int nmod,
                                   void* dwork, const void* ddesc, const int* dlut,
