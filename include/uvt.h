@@ -300,6 +300,9 @@ uvt_status uvt_malloc(uvt_handle h, size_t bytes, void **dptr);
 uvt_status uvt_free(uvt_handle h, void *dptr);
 /* kind: 0 host->device, 1 device->host, 2 device->device; synchronous. */
 uvt_status uvt_memcpy(uvt_handle h, void *dst, const void *src, size_t bytes, int32_t kind);
+/* Strided copy of `height` rows of `width` bytes (pitches in bytes). */
+uvt_status uvt_memcpy_2d(uvt_handle h, void *dst, size_t dpitch, const void *src, size_t spitch,
+                         size_t width, size_t height, int32_t kind);
 uvt_status uvt_memset(uvt_handle h, void *dptr, int32_t value, size_t bytes);
 uvt_status uvt_stream_create(uvt_handle h, void **stream);
 uvt_status uvt_stream_destroy(uvt_handle h, void *stream);

@@ -292,6 +292,28 @@ static void orc_blas_load(void)
     g_blas_state = 1;
 }
 
+/* Makes the CBLAS library at `path` the one the stage loop calls (a
+ * CBLAS-compatible replacement for MKL under test); NULL returns to the MKL
+ * search.  1 when both routines were found. */
+int orc_blas_use(const char *path)
+{
+    g_blas_state = 0;
+    g_dgemm = NULL;
+    g_dtrmm = NULL;
+    if (!path) return 1;
+    void *h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+    if (!h) return 0;
+    g_dgemm = (dgemm_fn)dlsym(h, "cblas_dgemm");
+    g_dtrmm = (dtrmm_fn)dlsym(h, "cblas_dtrmm");
+    if (!g_dgemm || !g_dtrmm) {
+        g_blas_state = -1;
+        return 0;
+    }
+    snprintf(g_blas_version, sizeof g_blas_version, "%s", path);
+    g_blas_state = 1;
+    return 1;
+}
+
 int orc_blas_available(char *version, int cap)
 {
     orc_blas_load();

@@ -70,6 +70,10 @@ int orc_stage_loop(int64_t n, int64_t m, uint32_t p, int balanced, const double 
                    const double *B, int64_t ldb, double *C, int64_t ldc, int blas);
 /* 1 if MKL's CBLAS could be loaded; fills a version string. */
 int orc_blas_available(char *version, int cap);
+/* Makes the CBLAS library at `path` the one orc_stage_loop calls (a
+ * CBLAS-compatible replacement under test); NULL returns to the MKL search.
+ * 1 when cblas_dgemm and cblas_dtrmm were found. */
+int orc_blas_use(const char *path);
 /* CPU baseline: `ranks` threads, each running the reference's stage loop
  * over its own column band of w columns (like one MPI rank with sequential
  * MKL).  Returns seconds of wall time for the timed loop; C gets the result. */

@@ -42,6 +42,7 @@ class Oracle:
         L.orc_trmm_rows_dd.argtypes = [i64, i64, P, i64, P, i64, c.POINTER(i64), i64, P, i64]
         L.orc_stage_loop.argtypes = [i64, i64, u32, i32, P, i64, P, i64, P, i64, i32]
         L.orc_blas_available.argtypes = [c.c_char_p, i32]
+        L.orc_blas_use.argtypes = [c.c_char_p]
         L.orc_ranks_stage_loop.argtypes = [i64, i64, u32, i32, i32, P, i64, P, i64, P, i64, i32]
         L.orc_ranks_stage_loop.restype = c.c_double
         L.orc_inf_norm.argtypes = [i64, i64, P, i64, P]

@@ -4,13 +4,14 @@ import ctypes
 import os
 import re
 import subprocess
+import sys
 import tempfile
 
 import pytest
 
 from conftest import ROOT
 
-HEADERS = ["uvt.h"]
+HEADERS = ["uvt.h", "uvt_cblas.h"]
 
 
 def declared(header):
@@ -69,3 +70,36 @@ def test_serve_request_bytes(uvt):
         uvt.serve_request_bytes(2000, 500, uvt.PREC_F64) + a(ws.value)
     with pytest.raises(uvt.UvtError):
         uvt.serve_request_bytes(0, 5, uvt.PREC_F64)
+
+
+def _cblas_subprocess(code):
+    """Runs a ctypes snippet against libuvt_cblas in a child process."""
+    prog = ("import ctypes, numpy as np\n"
+            f"L = ctypes.CDLL({os.path.join(ROOT, 'uvatrmm_amd', 'lib', 'libuvt_cblas.so')!r})\n"
+            "d, i = ctypes.c_double, ctypes.c_int\n" + code)
+    return subprocess.run([sys.executable, "-c", prog], capture_output=True, text=True, timeout=120)
+
+
+def test_cblas_illegal_parameters_return(uvt):
+    """An illegal parameter is reported by position (CBLAS numbering) and the
+    call returns with nothing written and no GPU touched, as MKL does."""
+    code = """
+C = np.full(16, 7.0); A = np.ones(16); B = np.ones(16)
+p = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+L.cblas_dgemm(101, 111, 111, -1, 2, 2, d(1), p(A), 2, p(B), 2, d(0), p(C), 2)
+L.cblas_dgemm(999, 111, 111, 2, 2, 2, d(1), p(A), 2, p(B), 2, d(0), p(C), 2)
+L.cblas_dgemm(101, 111, 111, 2, 3, 2, d(1), p(A), 2, p(B), 2, d(0), p(C), 3)
+L.cblas_dgemm(102, 112, 111, 2, 2, 3, d(1), p(A), 2, p(B), 3, d(0), p(C), 2)
+L.cblas_dtrmm(101, 141, 122, 111, 131, 4, 2, d(1), p(A), 3, p(B), 2)
+L.cblas_dtrmm(101, 141, 122, 111, 133, 2, 2, d(1), p(A), 2, p(B), 2)
+L.cblas_dtrmm(102, 142, 121, 111, 131, 3, 2, d(1), p(A), 2, p(B), 2)
+assert (C == 7.0).all() and (B == 1.0).all()
+print("ok")
+"""
+    r = _cblas_subprocess(code)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip() == "ok"
+    lines = [l for l in r.stderr.splitlines() if "libuvt_cblas" in l]
+    assert [l.split("parameter ")[1].split()[0] for l in lines] == \
+        ["4", "1", "11", "9", "10", "5", "12"], lines
+    assert all("cblas_dgemm" in l for l in lines[:4]) and all("cblas_dtrmm" in l for l in lines[4:])

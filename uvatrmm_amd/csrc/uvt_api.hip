@@ -441,6 +441,20 @@ extern "C" uvt_status uvt_memcpy(uvt_handle h, void* dst, const void* src, size_
     return UVT_OK;
 }
 
+extern "C" uvt_status uvt_memcpy_2d(uvt_handle h, void* dst, size_t dpitch, const void* src,
+                                    size_t spitch, size_t width, size_t height, int32_t kind)
+{
+    if (!h || kind < 0 || kind > 2) return UVT_ERR_ARG;
+    if (!width || !height) return UVT_OK;
+    if (!dst || !src || dpitch < width || spitch < width) return UVT_ERR_ARG;
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                            : kind == 1 ? hipMemcpyDeviceToHost
+                                        : hipMemcpyDeviceToDevice;
+    UVT_HIP(hipSetDevice(h->device));
+    UVT_HIP(hipMemcpy2D(dst, dpitch, src, spitch, width, height, k));
+    return UVT_OK;
+}
+
 extern "C" uvt_status uvt_memset(uvt_handle h, void* dptr, int32_t value, size_t bytes)
 {
     if (!h || (!dptr && bytes)) return UVT_ERR_ARG;
