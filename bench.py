@@ -29,6 +29,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
+# HBM bytes per launch of the FP64 tile kernel at N=32768, M=8192 (balanced,
+# trapezoid, 16 bands), from two rocprofv3 counter runs of this script:
+# 2.0 x FETCH_SIZE (139,086,503 KiB; the factor is what FETCH_SIZE reads for
+# k_row_abs_sums streaming exactly 2 GiB) + WRITE_SIZE (2,097,152 KiB).
+# Kept per workload; other workloads report null.
+MEASURED_TRAFFIC = {(32768, 8192, 16, "balanced", "trapezoid"): 2.0 * 139086503 * 1024 + 2097152 * 1024}
 
 
 def log(*a):
@@ -356,7 +362,8 @@ def main():
                    "n": n, "m": m, "bands": args.bands, "map": args.map, "shape": args.shape},
         "roofline": {"bound": "mfma", "kernel": kernel, "achieved": round(achieved, 4),
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                     "traffic": MEASURED_TRAFFIC.get((n, m, args.bands, args.map, args.shape)),
                      "launches_per_step": plan.phases,
                      "algorithmic_flops_per_step": flops},
         "cpu_baseline": cpu,

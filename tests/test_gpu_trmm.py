@@ -128,7 +128,7 @@ def test_generators_bit_exact(gpu, uvt, oracle):
     dB.free()
 
 
-@pytest.mark.parametrize("n", [257, 1000, 2048, 4096])
+@pytest.mark.parametrize("n", [257, 1000, 2048, 4096, 8192, 16384, 32768])
 def test_reference_norm_on_gpu(gpu, uvt, n):
     """Closed-form inputs generated on the GPU, product, norm: the reference's Norm line."""
     with open(os.path.join(GOLDEN, "reference_norms.json")) as f:
@@ -136,7 +136,9 @@ def test_reference_norm_on_gpu(gpu, uvt, n):
     dA, dB, dC = gpu.alloc(n * n * 8), gpu.alloc(n * n * 8), gpu.alloc(n * n * 8)
     say("norm case", n)
     gpu.fill_closed_form(n, n, dA, n, dB, n)
-    for mp, shape, p in (("reg", "boxes", 4), ("balanced", "trapezoid", 8)):
+    cases = (("reg", "boxes", 4), ("balanced", "trapezoid", 8)) if n <= 4096 else \
+        (("balanced", "trapezoid", 16),)
+    for mp, shape, p in cases:
         plan = uvt.Plan(n, n, p, mp, shape)
         gpu.trmm_f64(plan, dA, n, dB, n, dC, n)
         assert "%.3f" % gpu.inf_norm(n, n, dC, n) == want
@@ -231,3 +233,27 @@ def test_cli_reference_output(uvt):
     out = subprocess.run([uvt.CLI_PATH, "1000", "balanced", "trapezoid", "--bands", "3"],
                          capture_output=True, text=True, timeout=120, cwd=cwd)
     assert "\nNorm: 29.286\n" in out.stdout
+
+
+def test_cli_result_dump(uvt, oracle):
+    """--dump writes C in the reference's per-rank binary layout: header
+    {rows, columns, 0, rows, column begin, column count}, then the rank's
+    column slice row by row, the slices being map_regular over the columns."""
+    import tempfile
+    n, m, p = 300, 100, 3
+    say("cli dump", n, m, p)
+    cwd = tempfile.mkdtemp()
+    out = subprocess.run([uvt.CLI_PATH, str(n), "reg", "boxes", "--bands", str(p), "--m", str(m),
+                          "--dump", cwd], capture_output=True, text=True, timeout=120, cwd=cwd)
+    assert out.returncode == 0, out.stderr
+    ref = oracle.trmm(oracle.closed_A(n), oracle.closed_B(n, m))
+    begin = 0
+    for r in range(p):
+        raw = open(os.path.join(cwd, f"output_uvatrmm.{n}.{p}.{r}.dat"), "rb").read()
+        hdr = np.frombuffer(raw[:24], dtype=np.int32)
+        size = m // p + (m % p if r == p - 1 else 0)
+        assert hdr.tolist() == [n, m, 0, n, begin, size]
+        data = np.frombuffer(raw[24:], dtype=np.float64).reshape(n, size)
+        assert rel_fro(data, ref[:, begin:begin + size]) <= 1e-14
+        begin += size
+    assert begin == m

@@ -18,8 +18,9 @@
  *   --reps R         timed repetitions after one warm-up run (default 1)
  *   --device D       GPU ordinal (default 0)
  *   --json           also print one JSON line with the measurements
- *   --dump DIR       write C in the reference's binary result format
- *                    (DIR/output_uvatrmm.<N>.<P>.0.dat)
+ *   --dump DIR       write C in the reference's binary result format, one
+ *                    file per band's column slice
+ *                    (DIR/output_uvatrmm.<N>.<P>.<rank>.dat)
  */
 #define _POSIX_C_SOURCE 200809L
 #include <stdint.h>
@@ -68,18 +69,27 @@ static int fail(uvt_status s, const char *what)
     } while (0)
 
 /* Result dump in the reference's layout (print_output_results,
- * mm_triangular_common.c:2103-2132): int rows, int cols, int range[4]
- * = {0, rows, column begin, column count}, then rows*cols doubles. */
+ * mm_triangular_common.c:2103-2132), one file per virtual rank r of P:
+ * DIR/output_uvatrmm.<N>.<P>.<r>.dat holding int rows, int columns,
+ * int range[4] = {0, rows, column begin, column count} and then rows x count
+ * doubles, the rank's column band being map_regular over the columns
+ * (mm_triangular.c:90). */
 static int dump_result(const char *dir, long n, long m, unsigned p, const double *C)
 {
-    char path[4096];
-    snprintf(path, sizeof path, "%s/output_uvatrmm.%ld.%u.0.dat", dir, n, p);
-    FILE *f = fopen(path, "wb");
-    if (!f) return -1;
-    int hdr[6] = {(int)n, (int)m, 0, (int)n, 0, (int)m};
-    size_t ok = fwrite(hdr, sizeof(int), 6, f) == 6 &&
-                fwrite(C, sizeof(double), (size_t)n * (size_t)m, f) == (size_t)n * (size_t)m;
-    return (fclose(f) == 0 && ok) ? 0 : -1;
+    for (unsigned r = 0; r < p; r++) {
+        uvt_range cols;
+        if (uvt_map_band((uint64_t)m, p, r, UVT_MAP_REG, &cols, NULL) != UVT_OK) return -1;
+        char path[4096];
+        snprintf(path, sizeof path, "%s/output_uvatrmm.%ld.%u.%u.dat", dir, n, p, r);
+        FILE *f = fopen(path, "wb");
+        if (!f) return -1;
+        int hdr[6] = {(int)n, (int)m, 0, (int)n, (int)cols.begin, (int)cols.size};
+        int ok = fwrite(hdr, sizeof(int), 6, f) == 6;
+        for (long i = 0; ok && i < n && cols.size; i++)
+            ok = fwrite(C + (size_t)i * m + cols.begin, sizeof(double), cols.size, f) == cols.size;
+        if (fclose(f) != 0 || !ok) return -1;
+    }
+    return 0;
 }
 
 int main(int argc, char **argv)
