@@ -13,7 +13,7 @@ struct uvt_handle_s {
     int device;
     int cus;
     size_t vram;
-    void* work;        // grows on demand (norm row sums, split slices)
+    void* work;        // grows on demand (norm row sums, BLAS-form packing)
     size_t work_bytes;
 };
 

@@ -29,6 +29,8 @@
 // one; the batched runner (uvt_batch.hip) sends many.
 #include <hip/hip_runtime.h>
 #include <math.h>
+
+#include <algorithm>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -46,7 +48,8 @@ constexpr int kMaxSlices = 14;  // the two 64-bit halves of the reconstruction s
 constexpr int kTileI = 256;  // int8 tile rows = row-tile granularity of the box/diagonal split
 constexpr int kI8BK = 128;   // k bytes per LDS stage
 constexpr int kI8S = kI8BK + 16;  // LDS row stride (bytes): conflict-free 16-byte fragment reads
-constexpr int kI8Stage = 2 * kTileI * kI8S;
+constexpr int kI8Stage = kTileI * kI8S;  // one B stage
+constexpr int kFrag = 1024;       // one MFMA operand fragment: 32 rows x 32 k bytes
 
 // Pairwise coprime moduli, largest first (distinct prime factors).
 constexpr int kModuli[kMaxMods] = {256, 255, 253, 251, 247, 241, 239, 233,
@@ -57,14 +60,6 @@ struct Mods {
     int m[kMaxMods];
     double invd[kMaxMods];
     float invf[kMaxMods];
-    uint8_t prodmod[kMaxMods][kMaxMods];  // (m_0 ... m_{p-1}) mod m_l, p < l
-    uint8_t qinv[kMaxMods];               // (m_0 ... m_{l-1})^-1 mod m_l
-    uint32_t lift[kMaxMods];              // multiple of m_l >= 2^20 (keeps sums >= 0)
-    uint32_t magic[kMaxMods];             // ceil(2^39 / m_l): x mod m_l for x < 2^22
-    int split;                            // digits [0, split) form the low word
-    uint64_t wlo;                         // m_0 ... m_{split-1}   (< 2^56)
-    uint64_t qhi;                         // m_split ... m_{n-1}  (< 2^56)
-    double wlo_hi, wlo_lo;                // wlo as a double-word
 };
 
 // Stages, in launch order.
@@ -80,15 +75,16 @@ struct Desc {
     int* alpha;
     int* beta;
     double* part;
-    int8_t* Ar;               // [mod][n][kmax]
-    int8_t* Br;               // [mod][m][kmax] (B transposed)
+    int8_t* Ar;               // [mod][pad_tile(n)/32][kmax/32] fragments (see k_resid_a)
+    int8_t* Br;               // [mod][pad_tile(m)][kmax] (B transposed)
     uint8_t* R;               // [mod][n][m]
     double target;
     int32_t nrt, nct;         // 256-row/column tiles
     int32_t nrt128, nct128;   // 128 tiles of the FP64 diagonal part
-    int32_t gxa, gxr;         // workgroups per row in resid_a / reconstruct
+    int32_t gxa, gxr;         // resid_a workgroups per 32-row block (128 k each), reconstruct per row
     int32_t chunks;           // 256-row chunks of B's column norms
     int32_t vec;              // 16-byte FP64 loads allowed
+    int32_t gr;               // row tiles per XCD group of the int8 products (x 32/gr columns)
     int32_t start[K_NUM];     // first workgroup of this problem in each stage
     int32_t count[K_NUM];     // workgroups of this problem in each stage
 };
@@ -104,6 +100,9 @@ __device__ __forceinline__ Desc fetch(const Desc& one, const Desc* table, const 
 }
 
 __device__ __forceinline__ int64_t row_box_end(int64_t i) { return (i / kTileI) * kTileI; }
+// Residue arrays are padded with zero rows to whole 256-row tiles, so every
+// product tile loads without bounds checks.
+__host__ __device__ __forceinline__ int64_t pad_tile(int64_t x) { return (x + kTileI - 1) / kTileI * kTileI; }
 
 // ---- scales ---------------------------------------------------------------
 
@@ -170,33 +169,43 @@ __device__ __forceinline__ int sym_residue(double a, int m, double invm)
     return r;
 }
 
-// Ar[l][i][k] for k < kb(i), rows i >= 256: a workgroup takes 4096 k of a row,
-// 16 consecutive k per thread.
+// Rows of A'.  The residues go out in MFMA operand order: for row block
+// rb = i/32 and k block kb = k/32, fragment (rb, kb) is 1024 contiguous bytes
+// where lane l = i%32 + 32*((k%32)/16) holds k bytes 16*(l/32) .. +16 of row
+// rb*32 + l%32, i.e. the 16 bytes lane l feeds v_mfma_i32_32x32x32_i8.  A
+// wave of the product kernel then loads a fragment with one coalesced
+// 1 KiB read.  A workgroup covers 32 rows (rows >= 256) x 128 k, i.e. four
+// fragments; thread t takes 16 consecutive k of row t/8, so a wave reads 8
+// rows x 1 KiB of A and writes 8 whole 128-byte lines per modulus.  Rows
+// n .. pad_tile(n) are zeros.
 __global__ __launch_bounds__(256) void k_resid_a(Desc one, const Desc* table, const int* lut,
                                                  int nprob, Mods mods)
 {
     const Desc d = fetch(one, table, lut, nprob);
     const int64_t wb = (int64_t)blockIdx.x - d.start[K_RA];
-    const int64_t i = kTileI + wb / d.gxa;
-    const int64_t k0 = (wb % d.gxa) * 4096 + threadIdx.x * 16;
-    if (i >= d.n || k0 >= row_box_end(i)) return;  // kb(i) is a multiple of 16
-    const int al = d.alpha[i];
+    const int64_t rb = kTileI / 32 + wb / d.gxa, kq = wb % d.gxa;
+    if (kq * 128 >= row_box_end(rb * 32)) return;  // boxes end on multiples of 256
+    const int t = threadIdx.x, c = t & 7;
+    const int64_t i = rb * 32 + (t >> 3);
+    const int64_t k0 = kq * 128 + c * 16;
+    const bool iv = i < d.n;
+    const int al = iv ? d.alpha[i] : 0;
     double v[16];
     const double* a = d.A + i * d.lda + k0;
 #pragma unroll
-    for (int e = 0; e < 16; ++e) v[e] = rint(ldexp(a[e], al));
-    int8_t* out = d.Ar + i * d.kmax + k0;
-    const int64_t mod_stride = d.n * d.kmax;
+    for (int e = 0; e < 16; ++e) v[e] = iv ? rint(ldexp(a[e], al)) : 0.0;
+    const int64_t kbn = d.kmax / 32, mod_stride = pad_tile(d.n) * d.kmax;
+    int8_t* out = d.Ar + ((rb * kbn + kq * 4 + (c >> 1)) * 64 + (t >> 3) + 32 * (c & 1)) * 16;
     for (int l = 0; l < mods.n; ++l) {
         uint32_t q4[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int w = 0; w < 4; ++w) {
             uint32_t x = 0;
 #pragma unroll
             for (int e = 0; e < 4; ++e)
-                x |= (uint32_t)(uint8_t)(int8_t)sym_residue(v[q * 4 + e], mods.m[l], mods.invd[l])
+                x |= (uint32_t)(uint8_t)(int8_t)sym_residue(v[w * 4 + e], mods.m[l], mods.invd[l])
                      << (8 * e);
-            q4[q] = x;
+            q4[w] = x;
         }
         *reinterpret_cast<v4i*>(out + l * mod_stride) =
             v4i{(int)q4[0], (int)q4[1], (int)q4[2], (int)q4[3]};
@@ -204,6 +213,7 @@ __global__ __launch_bounds__(256) void k_resid_a(Desc one, const Desc* table, co
 }
 
 // Br[l][j][k] (B transposed) for k < kmax: 64 x 64 tiles through LDS.
+// Rows m .. pad_tile(m) are zeros.
 __global__ __launch_bounds__(256) void k_resid_b(Desc one, const Desc* table, const int* lut,
                                                  int nprob, Mods mods)
 {
@@ -223,13 +233,13 @@ __global__ __launch_bounds__(256) void k_resid_b(Desc one, const Desc* table, co
         v[q] = jv ? rint(ldexp(d.B[k * d.ldb + j], be)) : 0.0;
     }
     const int wr = threadIdx.x >> 2, ws = (threadIdx.x & 3) * 16;
-    const int64_t mod_stride = d.m * d.kmax;
+    const int64_t mod_stride = pad_tile(d.m) * d.kmax;
     for (int l = 0; l < mods.n; ++l) {
 #pragma unroll
         for (int q = 0; q < 16; ++q)
             t[jj][kq + 4 * q] = (int8_t)sym_residue(v[q], mods.m[l], mods.invd[l]);
         __syncthreads();
-        if (j0 + wr < d.m) {
+        {
             const v4i x = *reinterpret_cast<const v4i*>(&t[wr][ws]);
             *reinterpret_cast<v4i*>(d.Br + l * mod_stride + (j0 + wr) * d.kmax + k0 + ws) = x;
         }
@@ -239,67 +249,78 @@ __global__ __launch_bounds__(256) void k_resid_b(Desc one, const Desc* table, co
 
 // ---- int8 modular products ---------------------------------------------------
 
-// The k loop of one 256 x 256 tile: A' and B' residue panels (128 bytes of k
-// per stage) through a double-buffered LDS tile; 8 waves of 64 x 128, each a
-// 2 x 4 block of v_mfma_i32_32x32x32_i8.  FULL: the tile lies inside C, so the
-// loads need no bounds checks.
-template <bool FULL>
-__device__ __forceinline__ void i8_tile_loop(const int8_t* __restrict__ A, int64_t lda_r,
-                                             const int8_t* __restrict__ B, int64_t ldb_r, int64_t rows_left,
-                                             int64_t cols_left, int K, int8_t* smem, v16i (&acc)[2][4])
+// The k loop of one 256 x 256 tile; 8 waves of 64 x 128, each a 2 x 4 block
+// of v_mfma_i32_32x32x32_i8.  A' fragments come straight from memory into
+// registers (fragment order, one coalesced 1 KiB load each), one k stage
+// (128 bytes) ahead; B' panels go through a double-buffered LDS tile.  The
+// loop is unrolled by two so both register buffers have fixed names.  The
+// residue arrays are padded to whole tiles, so no load needs a bounds check.
+__device__ __forceinline__ void i8_tile_loop(const int8_t* __restrict__ Af, int64_t kbn,
+                                             const int8_t* __restrict__ B, int64_t ldb_r, int K,
+                                             int8_t* smem, v16i (&acc)[2][4])
 {
+    constexpr int KS = kI8BK / 32;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    v4i ra[4], rb[4];
-    auto load = [&](int k0) {
+    // this lane's 16 bytes of fragment (row block 2*wm + i, k block kb)
+    const int8_t* Aw = Af + (int64_t)(2 * wm) * kbn * kFrag + lane * 16;
+    v4i a0[KS][2], a1[KS][2], rb[4];
+    auto load_a = [&](int k0, v4i (&d)[KS][2]) {
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                d[ks][i] = *reinterpret_cast<const v4i*>(Aw + ((int64_t)i * kbn + (k0 >> 5) + ks) * kFrag);
+    };
+    auto load_b = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int c = tid + q * 512;
             const int row = c >> 3, seg = (c & 7) * 16;
-            if (FULL || row < rows_left) ra[q] = *reinterpret_cast<const v4i*>(A + row * lda_r + k0 + seg);
-            else ra[q] = v4i{0};
-            if (FULL || row < cols_left) rb[q] = *reinterpret_cast<const v4i*>(B + row * ldb_r + k0 + seg);
-            else rb[q] = v4i{0};
+            rb[q] = *reinterpret_cast<const v4i*>(B + row * ldb_r + k0 + seg);
         }
     };
-    auto store = [&](int buf) {
-        int8_t* as = smem + buf * kI8Stage;
-        int8_t* bs = as + kTileI * kI8S;
+    auto store_b = [&](int buf) {
+        int8_t* bs = smem + buf * kI8Stage;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int c = tid + q * 512;
             const int row = c >> 3, seg = (c & 7) * 16;
-            *reinterpret_cast<v4i*>(as + row * kI8S + seg) = ra[q];
             *reinterpret_cast<v4i*>(bs + row * kI8S + seg) = rb[q];
         }
     };
-    auto compute = [&](int buf) {
-        const int8_t* as = smem + buf * kI8Stage + (wm * 64 + (lane & 31)) * kI8S + (lane >> 5) * 16;
-        const int8_t* bs = smem + buf * kI8Stage + kTileI * kI8S + (wn * 128 + (lane & 31)) * kI8S +
-                           (lane >> 5) * 16;
+    auto compute = [&](int buf, v4i (&a)[KS][2]) {
+        const int8_t* bs = smem + buf * kI8Stage + (wn * 128 + (lane & 31)) * kI8S + (lane >> 5) * 16;
 #pragma unroll
-        for (int ks = 0; ks < kI8BK / 32; ++ks) {
-            v4i a[2], b[4];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const v4i*>(as + i * 32 * kI8S + ks * 32);
+        for (int ks = 0; ks < KS; ++ks) {
+            v4i b[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const v4i*>(bs + j * 32 * kI8S + ks * 32);
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[ks][i], b[j], acc[i][j], 0, 0, 0);
         }
     };
-    const int nk = K / kI8BK;  // K is a multiple of 256
-    load(0);
-    store(0);
+    const int nk = K / kI8BK;  // K is a multiple of 256, so nk is even
+    load_a(0, a0);
+    load_b(0);
+    store_b(0);
     __syncthreads();
-    for (int kt = 0; kt < nk; ++kt) {
-        const bool more = kt + 1 < nk;
-        if (more) load((kt + 1) * kI8BK);
-        compute(kt & 1);
-        if (more) store((kt + 1) & 1);
+    for (int kt = 0; kt < nk; kt += 2) {
+        load_a((kt + 1) * kI8BK, a1);
+        load_b((kt + 1) * kI8BK);
+        compute(0, a0);
+        store_b(1);
+        __syncthreads();
+        const bool more = kt + 2 < nk;
+        if (more) {
+            load_a((kt + 2) * kI8BK, a0);
+            load_b((kt + 2) * kI8BK);
+        }
+        compute(1, a1);
+        if (more) store_b(0);
         __syncthreads();
     }
 }
@@ -308,7 +329,7 @@ __device__ __forceinline__ void i8_tile_loop(const int8_t* __restrict__ A, int64
 __global__ __launch_bounds__(512, 1) void k_gemm_i8(Desc one, const Desc* table, const int* lut,
                                                     int nprob, Mods mods)
 {
-    __shared__ __attribute__((aligned(16))) int8_t smem[2 * kI8Stage];
+    __shared__ __attribute__((aligned(16))) int8_t smem[2 * kI8Stage];  // B' only
     // Only the fields used here are read (from the kernel argument or the
     // descriptor table), so the descriptor does not occupy registers.
     const Desc* dp = nprob == 1 ? nullptr : &table[lut[blockIdx.x]];
@@ -317,21 +338,24 @@ __global__ __launch_bounds__(512, 1) void k_gemm_i8(Desc one, const Desc* table,
     // XCD-aware mapping.  Workgroups go to the 8 XCDs round-robin (every
     // problem starts at a multiple of 256 workgroups), so XCD x runs local
     // blocks x, x+8, ...; its s-th block takes tile s%32 of tile group
-    // (s/32)*8 + x.  A group is 4 row tiles x 8 column tiles of one modulus,
-    // i.e. the 32 workgroups one XCD holds at once, so they share 4 A panels
-    // and 8 B panels through that XCD's L2.  Groups run longest-k first.
+    // (s/32)*8 + x.  A group is gr row tiles x 32/gr column tiles of one
+    // modulus (8 x 4 for large problems), i.e. the 32 workgroups one XCD holds
+    // at once, so they share their A and B panels through that XCD's L2.
+    // Groups run longest-k first.
     const int xcd = (int)(wb & 7), slot = (int)(wb >> 3);
     const int g = (slot >> 5) * 8 + xcd, t = slot & 31;
     const int nct = UVT_FIELD(nct), nrt = UVT_FIELD(nrt);
-    const int ngc = (nct + 7) / 8, per_rg = ngc * mods.n;
+    const int gr = UVT_FIELD(gr), gc = 32 / gr;
+    const int ngc = (nct + gc - 1) / gc, per_rg = ngc * mods.n;
     const int rg = g / per_rg, grem = g % per_rg;
-    const int l = grem / ngc, ct = (grem % ngc) * 8 + (t & 7);
-    const int rt = nrt - 1 - (rg * 4 + (t >> 3));
+    const int l = grem / ngc, ct = (grem % ngc) * gc + t % gc;
+    const int rt = nrt - 1 - (rg * gr + t / gc);
     if (rt < 1 || ct >= nct) return;  // padding of the last groups (uniform per block)
     const int64_t r0 = (int64_t)rt * kTileI, c0 = (int64_t)ct * kTileI;
     const int64_t n = UVT_FIELD(n), m = UVT_FIELD(m), kmax = UVT_FIELD(kmax);
-    const int8_t* A = UVT_FIELD(Ar) + l * (n * kmax) + r0 * kmax;
-    const int8_t* B = UVT_FIELD(Br) + l * (m * kmax) + c0 * kmax;
+    const int64_t kbn = kmax / 32;
+    const int8_t* Af = UVT_FIELD(Ar) + l * (pad_tile(n) * kmax) + (r0 / 32) * kbn * kFrag;
+    const int8_t* B = UVT_FIELD(Br) + l * (pad_tile(m) * kmax) + c0 * kmax;
     uint8_t* const Rout = UVT_FIELD(R);
 #undef UVT_FIELD
 
@@ -340,10 +364,7 @@ __global__ __launch_bounds__(512, 1) void k_gemm_i8(Desc one, const Desc* table,
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = v16i{0};
-    if (r0 + kTileI <= n && c0 + kTileI <= m)
-        i8_tile_loop<true>(A, kmax, B, kmax, n - r0, m - c0, (int)r0, smem, acc);
-    else
-        i8_tile_loop<false>(A, kmax, B, kmax, n - r0, m - c0, (int)r0, smem, acc);
+    i8_tile_loop(Af, kbn, B, kmax, (int)r0, smem, acc);
 
     // D layout of v_mfma_i32_32x32x32_i8 (measured on gfx950): lane l, element
     // v holds row 8*(v/4) + 4*(l/32) + v%4, column l%32.
@@ -408,16 +429,78 @@ __device__ __forceinline__ uint32_t mod_magic(uint32_t x, uint32_t m, uint32_t m
     return x - (__umulhi(x, magic) >> 7) * m;
 }
 
-// C[i][j] += C'_ij * 2^-(alpha_i + beta_j) for rows past the first 256-row
-// tile.  Per element: mixed-radix digits v_l of C' from its residues
-//   v_l = (r_l - sum_{p<l} v_p (m_0...m_{p-1} mod m_l)) (m_0...m_{l-1})^-1 mod m_l,
-// then x = X_lo + W X_hi with X_lo = digits [0, split) and X_hi = the rest,
-// both exact in 64 bits, and the signed value (x, or x - M when x > M/2)
-// rounded once to FP64 with a double-word W.  A workgroup takes 1024 columns
-// of a row, four per thread.
-__global__ __launch_bounds__(256) void k_reconstruct(Desc one, const Desc* table, const int* lut,
-                                                     int nprob, Mods mods)
+// Garner tables for the first N moduli, evaluated at compile time so that the
+// reconstruction kernel sees every modulus and coefficient as an immediate.
+// With W_p = m_0 ... m_{p-1}, the mixed-radix digits of x (x = sum_p v_p W_p)
+// satisfy  v_l = (r_l q_l + sum_{p<l} v_p c_{p,l}) mod m_l  where
+// q_l = W_l^-1 mod m_l and c_{p,l} = m_l - (W_p q_l mod m_l): one modular
+// reduction per digit, and every partial sum stays below 14 * 255^2 < 2^20.
+template <int N>
+struct GarnerTab {
+    static constexpr int split = N - 1 < 7 ? N - 1 : 7;  // digits [0, split) form the low word
+    uint32_t q[N];
+    uint32_t c[N][N];
+    uint32_t magic[N];  // ceil(2^39 / m_l): x mod m_l for x < 2^22
+    uint64_t wlo, qhi;  // m_0 ... m_{split-1};  m_split ... m_{N-1}
+    double wlo_hi, wlo_lo;
+    constexpr GarnerTab() : q(), c(), magic(), wlo(1), qhi(1), wlo_hi(0.0), wlo_lo(0.0)
+    {
+        for (int l = 0; l < N; ++l) {
+            const uint32_t ml = (uint32_t)kModuli[l];
+            uint32_t wp[N] = {};
+            uint32_t w = 1 % ml;  // W_p mod m_l
+            for (int p = 0; p < l; ++p) {
+                wp[p] = w;
+                w = (w * (uint32_t)kModuli[p]) % ml;
+            }
+            uint32_t inv = 1;
+            for (uint32_t x = 1; x < ml; ++x)
+                if ((w * x) % ml == 1) {
+                    inv = x;
+                    break;
+                }
+            q[l] = inv;
+            for (int p = 0; p < l; ++p) c[p][l] = ml - (wp[p] * inv) % ml;
+            magic[l] = (uint32_t)(((1ull << 39) + ml - 1) / ml);
+        }
+        for (int l = 0; l < split; ++l) wlo *= (uint64_t)kModuli[l];
+        for (int l = split; l < N; ++l) qhi *= (uint64_t)kModuli[l];
+        wlo_hi = (double)wlo;
+        wlo_lo = (double)((int64_t)wlo - (int64_t)(uint64_t)wlo_hi);
+    }
+};
+
+// sum_{l in [A, B)} v_l m_A ... m_{l-1} (exact; < 2^56 for B - A <= 7):
+// Horner over chunks of three digits, each chunk exact in 24 bits.
+template <int A, int B>
+__device__ __forceinline__ uint64_t digits_value(const uint32_t* v)
 {
+    constexpr int nch = (B - A + 2) / 3;
+    uint64_t w = 0;
+#pragma unroll
+    for (int ch = nch - 1; ch >= 0; --ch) {
+        const int lo = A + 3 * ch, hi = lo + 3 < B ? lo + 3 : B;
+        uint32_t cv = 0, P = 1;
+#pragma unroll
+        for (int l = hi - 1; l >= lo; --l) cv = __umul24(cv, (uint32_t)kModuli[l]) + v[l];
+#pragma unroll
+        for (int l = lo; l < hi; ++l) P *= (uint32_t)kModuli[l];
+        w = w * P + cv;
+    }
+    return w;
+}
+
+// C[i][j] += C'_ij * 2^-(alpha_i + beta_j) for rows past the first 256-row
+// tile, with N moduli.  Per element: the mixed-radix digits of C' from its
+// residues (GarnerTab), then x = X_lo + W X_hi with X_lo = digits [0, split)
+// and X_hi = the rest, both exact in 64 bits, and the signed value (x, or
+// x - M when x > M/2) rounded once to FP64 with a double-word W.  A
+// workgroup takes 1024 columns of a row, four per thread.
+template <int N>
+__global__ __launch_bounds__(256) void k_reconstruct(Desc one, const Desc* table, const int* lut,
+                                                     int nprob)
+{
+    constexpr GarnerTab<N> G{};
     const Desc d = fetch(one, table, lut, nprob);
     const int64_t wb = (int64_t)blockIdx.x - d.start[K_REC];
     const int64_t i = kTileI + wb / d.gxr;
@@ -427,72 +510,58 @@ __global__ __launch_bounds__(256) void k_reconstruct(Desc one, const Desc* table
     const int ne = m - j4 < 4 ? (int)(m - j4) : 4;
     const bool vec4 = (m & 3) == 0 && ne == 4;
     const int64_t mod_stride = d.n * m;
-    uint32_t res[kMaxMods][4];
+    uint32_t res[N][4];
 #pragma unroll
-    for (int l = 0; l < kMaxMods; ++l) {
-        if (l < mods.n) {
-            const uint8_t* r = d.R + l * mod_stride + i * m + j4;
-            if (vec4) {
-                const uint32_t x = *reinterpret_cast<const uint32_t*>(r);
+    for (int l = 0; l < N; ++l) {
+        const uint8_t* r = d.R + l * mod_stride + i * m + j4;
+        if (vec4) {
+            const uint32_t x = *reinterpret_cast<const uint32_t*>(r);
 #pragma unroll
-                for (int e = 0; e < 4; ++e) res[l][e] = (x >> (8 * e)) & 255u;
-            } else {
-#pragma unroll
-                for (int e = 0; e < 4; ++e) res[l][e] = e < ne ? r[e] : 0u;
-            }
+            for (int e = 0; e < 4; ++e) res[l][e] = (x >> (8 * e)) & 255u;
         } else {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) res[l][e] = 0u;
+            for (int e = 0; e < 4; ++e) res[l][e] = e < ne ? r[e] : 0u;
         }
     }
     const int ai = d.alpha[i];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         if (e >= ne) break;
-        uint32_t v[kMaxMods];
+        uint32_t v[N];
 #pragma unroll
-        for (int l = 0; l < kMaxMods; ++l) {
-            uint32_t t = 0;
-            if (l < mods.n) {
-                uint32_t acc = mods.lift[l] + res[l][e];
+        for (int l = 0; l < N; ++l) {
+            uint32_t acc = __umul24(res[l][e], G.q[l]);
 #pragma unroll
-                for (int p = 0; p < l; ++p) acc -= v[p] * (uint32_t)mods.prodmod[p][l];
-                t = mod_magic(acc, (uint32_t)mods.m[l], mods.magic[l]);
-                t = mod_magic(t * (uint32_t)mods.qinv[l], (uint32_t)mods.m[l], mods.magic[l]);
-            }
-            v[l] = t;
+            for (int p = 0; p < l; ++p) acc += __umul24(v[p], G.c[p][l]);
+            v[l] = kModuli[l] == 256 ? (acc & 255u)
+                                     : mod_magic(acc, (uint32_t)kModuli[l], G.magic[l]);
         }
-        uint64_t xlo = 0, xhi = 0;
-#pragma unroll
-        for (int l = kMaxMods - 1; l >= 0; --l) {
-            if (l < mods.split) xlo = xlo * (uint64_t)mods.m[l] + v[l];
-            else if (l < mods.n) xhi = xhi * (uint64_t)mods.m[l] + v[l];
-        }
+        const uint64_t xlo = digits_value<0, GarnerTab<N>::split>(v);
+        const uint64_t xhi = digits_value<GarnerTab<N>::split, N>(v);
         double h, low;
-        if (2 * xhi > mods.qhi) {  // x > M/2: value x - M = (X_hi - Q + 1) W + (X_lo - W)
-            h = (double)((int64_t)xhi - (int64_t)mods.qhi + 1);
-            low = (double)((int64_t)xlo - (int64_t)mods.wlo);
+        if (2 * xhi > G.qhi) {  // x > M/2: value x - M = (X_hi - Q + 1) W + (X_lo - W)
+            h = (double)((int64_t)xhi - (int64_t)G.qhi + 1);
+            low = (double)((int64_t)xlo - (int64_t)G.wlo);
         } else {
             h = (double)xhi;
             low = (double)xlo;
         }
-        const double x = fma(h, mods.wlo_hi, fma(h, mods.wlo_lo, low));
+        const double x = fma(h, G.wlo_hi, fma(h, G.wlo_lo, low));
         const int64_t j = j4 + e;
         d.C[i * d.ldc + j] += ldexp(x, -(ai + d.beta[j]));
     }
 }
 
+template <int N>
+void launch_reconstruct(unsigned grid, hipStream_t s, const Desc& one, const Desc* table,
+                        const int* lut, int nprob)
+{
+    hipLaunchKernelGGL(k_reconstruct<N>, dim3(grid), dim3(256), 0, s, one, table, lut, nprob);
+}
+
 // ---- host side ------------------------------------------------------------------
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
-int inv_mod(int a, int m)
-{
-    a %= m;
-    for (int x = 1; x < m; ++x)
-        if ((a * x) % m == 1) return x;
-    return 0;
-}
 
 Mods make_mods(int nmod, double* log2M)
 {
@@ -504,27 +573,27 @@ Mods make_mods(int nmod, double* log2M)
         md.m[l] = kModuli[l];
         md.invf[l] = 1.0f / (float)kModuli[l];
         md.invd[l] = 1.0 / (double)kModuli[l];
-        md.magic[l] = (uint32_t)(((1ull << 39) + kModuli[l] - 1) / kModuli[l]);
         lg += log2((double)kModuli[l]);
     }
-    for (int l = 0; l < nmod; ++l) {
-        int w = 1;  // (m_0 ... m_{p-1}) mod m_l
-        for (int p = 0; p < l; ++p) {
-            md.prodmod[p][l] = (uint8_t)w;
-            w = (w * md.m[p]) % md.m[l];
-        }
-        md.qinv[l] = (uint8_t)inv_mod(w, md.m[l]);
-        md.lift[l] = (uint32_t)(((1 << 20) / md.m[l] + 1) * md.m[l]);
-    }
-    md.split = nmod - 1 < 7 ? nmod - 1 : 7;
-    md.wlo = 1;
-    for (int l = 0; l < md.split; ++l) md.wlo *= (uint64_t)md.m[l];
-    md.qhi = 1;
-    for (int l = md.split; l < nmod; ++l) md.qhi *= (uint64_t)md.m[l];
-    md.wlo_hi = (double)md.wlo;
-    md.wlo_lo = (double)((int64_t)md.wlo - (int64_t)(uint64_t)md.wlo_hi);
     *log2M = lg;
     return md;
+}
+
+// Row tiles per XCD group of the int8 products.  8 x 4 groups share panels
+// best through an XCD's L2 (measured); another shape is taken only when it
+// leaves at least 15 % fewer group slots over rtiles x ctiles tiles.
+int group_rows(int64_t rtiles, int64_t ctiles)
+{
+    static const int pref[] = {8, 4, 16, 2, 32, 1};
+    auto slots = [&](int gr) {
+        const int64_t gc = 32 / gr;
+        return (rtiles + gr - 1) / gr * ((ctiles + gc - 1) / gc);
+    };
+    int64_t best = INT64_MAX;
+    for (int gr : pref) best = std::min(best, slots(gr));
+    for (int gr : pref)
+        if (slots(gr) * 100 <= best * 115) return gr;
+    return 8;
 }
 
 // Workspace bytes of one problem (scales, partial norms, residues).
@@ -537,8 +606,8 @@ size_t problem_bytes(int64_t n, int64_t m, int nmod)
     o = align256(o + (size_t)n * 4);
     o = align256(o + (size_t)m * 4);
     o = align256(o + (size_t)chunks * m * 8);
-    o = align256(o + (size_t)nmod * n * kmax);
-    o = align256(o + (size_t)nmod * m * kmax);
+    o = align256(o + (size_t)nmod * pad_tile(n) * kmax);
+    o = align256(o + (size_t)nmod * pad_tile(m) * kmax);
     o = align256(o + (size_t)nmod * n * m);
     return o;
 }
@@ -580,7 +649,7 @@ void describe(const uvt_split_problem* probs, int count, int nmod, char* work, D
         d.nct = (int32_t)((q.m + kTileI - 1) / kTileI);
         d.nrt128 = (int32_t)((q.n + 127) / 128);
         d.nct128 = (int32_t)((q.m + 127) / 128);
-        d.gxa = (int32_t)((d.kmax + 4095) / 4096);
+        d.gxa = (int32_t)(d.kmax / 128);
         d.gxr = (int32_t)((q.m + 1023) / 1024);
         d.chunks = (int32_t)(d.kmax / 256);
         d.vec = ((uintptr_t)q.A % 16 == 0) && ((uintptr_t)q.B % 16 == 0) && q.lda % 2 == 0 &&
@@ -595,18 +664,20 @@ void describe(const uvt_split_problem* probs, int count, int nmod, char* work, D
             d.part = (double*)(w + o);
             o = align256(o + (size_t)d.chunks * q.m * 8);
             d.Ar = (int8_t*)(w + o);
-            o = align256(o + (size_t)nmod * q.n * d.kmax);
+            o = align256(o + (size_t)nmod * pad_tile(q.n) * d.kmax);
             d.Br = (int8_t*)(w + o);
-            o = align256(o + (size_t)nmod * q.m * d.kmax);
+            o = align256(o + (size_t)nmod * pad_tile(q.m) * d.kmax);
             d.R = (uint8_t*)(w + o);
             o = align256(o + (size_t)nmod * q.n * q.m);
             off += o;
             d.count[K_ALPHA] = (int32_t)((q.n + 3) / 4);
             d.count[K_BPART] = (int32_t)(((q.m + 255) / 256) * d.chunks);
             d.count[K_BETA] = (int32_t)((q.m + 255) / 256);
-            d.count[K_RA] = (int32_t)((q.n - kTileI) * d.gxa);
-            d.count[K_RB] = (int32_t)((d.kmax / 64) * ((q.m + 63) / 64));
-            const int64_t groups = (int64_t)((d.nrt - 1 + 3) / 4) * ((d.nct + 7) / 8) * nmod;
+            d.count[K_RA] = (int32_t)((pad_tile(q.n) - kTileI) / 32 * d.gxa);
+            d.count[K_RB] = (int32_t)((d.kmax / 64) * (pad_tile(q.m) / 64));
+            d.gr = group_rows(d.nrt - 1, d.nct);
+            const int64_t groups =
+                (int64_t)((d.nrt - 1 + d.gr - 1) / d.gr) * ((d.nct + 32 / d.gr - 1) / (32 / d.gr)) * nmod;
             d.count[K_GEMM] = (int32_t)((groups + 7) / 8 * 8 * 32);  // multiple of 256
             d.count[K_REC] = (int32_t)((q.n - kTileI) * d.gxr);
         }
@@ -706,8 +777,21 @@ uvt_status uvt_split_batch_launch(const uvt_split_problem* single, int count, in
         hipLaunchKernelGGL(k_gemm_i8, dim3((unsigned)totals[K_GEMM]), dim3(512), 0, s, one, table,
                            lut(K_GEMM), count, mods);
     if ((stages & UVT_SPLIT_REC) && totals[K_REC])
-        hipLaunchKernelGGL(k_reconstruct, dim3((unsigned)totals[K_REC]), dim3(256), 0, s, one, table,
-                           lut(K_REC), count, mods);
+        switch (nmod) {
+        case 2: launch_reconstruct<2>((unsigned)totals[K_REC], s, one, table, lut(K_REC), count); break;
+        case 3: launch_reconstruct<3>((unsigned)totals[K_REC], s, one, table, lut(K_REC), count); break;
+        case 4: launch_reconstruct<4>((unsigned)totals[K_REC], s, one, table, lut(K_REC), count); break;
+        case 5: launch_reconstruct<5>((unsigned)totals[K_REC], s, one, table, lut(K_REC), count); break;
+        case 6: launch_reconstruct<6>((unsigned)totals[K_REC], s, one, table, lut(K_REC), count); break;
+        case 7: launch_reconstruct<7>((unsigned)totals[K_REC], s, one, table, lut(K_REC), count); break;
+        case 8: launch_reconstruct<8>((unsigned)totals[K_REC], s, one, table, lut(K_REC), count); break;
+        case 9: launch_reconstruct<9>((unsigned)totals[K_REC], s, one, table, lut(K_REC), count); break;
+        case 10: launch_reconstruct<10>((unsigned)totals[K_REC], s, one, table, lut(K_REC), count); break;
+        case 11: launch_reconstruct<11>((unsigned)totals[K_REC], s, one, table, lut(K_REC), count); break;
+        case 12: launch_reconstruct<12>((unsigned)totals[K_REC], s, one, table, lut(K_REC), count); break;
+        case 13: launch_reconstruct<13>((unsigned)totals[K_REC], s, one, table, lut(K_REC), count); break;
+        default: launch_reconstruct<14>((unsigned)totals[K_REC], s, one, table, lut(K_REC), count); break;
+        }
     UVT_HIP(hipGetLastError());
     return UVT_OK;
 }
@@ -735,6 +819,9 @@ extern "C" uvt_status uvt_trmm_split(uvt_handle h, int64_t n, int64_t m, int32_t
     if (st != UVT_OK) return st;
     if (!dwork || work_bytes < need || ((uintptr_t)dwork & 255)) return UVT_ERR_ARG;
     UVT_HIP(hipSetDevice(h->device));
+    // One stream, stage after stage: running the FP64 diagonal blocks beside
+    // the HBM-bound scale and residue stages saves nothing (measured), and
+    // the int8 products want every CU to themselves.
     const int64_t zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     return uvt_split_batch_launch(&p, 1, slices, dwork, nullptr, nullptr, zero, zero,
                                   (hipStream_t)stream, UVT_SPLIT_ALL);
