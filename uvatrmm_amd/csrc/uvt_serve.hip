@@ -104,10 +104,6 @@ struct Pending {
     size_t bytes;
 };
 
-struct Block {
-    size_t off, bytes;
-};
-
 struct Slot {
     bool busy = false;
     hipStream_t stream = nullptr;
@@ -115,7 +111,7 @@ struct Slot {
     Pending p{};
     double t_admit = 0.0;
     double* dC = nullptr;
-    std::vector<Block> blocks;
+    size_t off = SIZE_MAX;  // the request's arena block (p.bytes long)
     uvt_status status = UVT_OK;
 };
 
@@ -175,30 +171,31 @@ extern "C" uvt_status uvt_serve_request_bytes(const uvt_serve_config* cfg, int64
     return UVT_OK;
 }
 
-// Enqueues one admitted request on its slot (scheduler thread, lock not held).
+// Enqueues one admitted request on its slot (scheduler thread, lock not
+// held).  Its operands are carved, in request_bytes order, out of the one
+// arena block admission reserved for it.
 static uvt_status launch(uvt_server s, Slot& sl)
 {
     const uvt_request& r = sl.p.r;
     const int64_t n = r.n, m = r.m;
-    auto block = [&](size_t bytes) -> char* {
-        const size_t off = s->arena.alloc(bytes);
-        if (off == SIZE_MAX) return nullptr;
-        sl.blocks.push_back(Block{off, bytes});
-        return s->arena.at(off);
+    char* cursor = s->arena.at(sl.off);
+    auto carve = [&](size_t bytes) -> char* {
+        char* p = cursor;
+        cursor += align_up(bytes);
+        return p;
     };
-    // admission reserved the total; the blocks below come out of it
     const bool own_ab = ab_in_arena(r);
-    const double* dA = own_ab ? (const double*)block((size_t)n * n * 8) : r.A;
-    const double* dB = own_ab ? (const double*)block((size_t)n * m * 8) : r.B;
-    double* dC = c_in_arena(r) ? (double*)block((size_t)n * m * 8) : r.C;
+    const double* dA = own_ab ? (const double*)carve((size_t)n * n * 8) : r.A;
+    const double* dB = own_ab ? (const double*)carve((size_t)n * m * 8) : r.B;
+    double* dC = c_in_arena(r) ? (double*)carve((size_t)n * m * 8) : r.C;
     const bool int8 = r.prec == UVT_PREC_SPLIT && split_pays_off(n, m);
     size_t wbytes = 0;
     void* work = nullptr;
     if (int8) {
         if (uvt_trmm_split_workspace(n, m, s->cfg.slices, &wbytes) != UVT_OK) return UVT_ERR_ARG;
-        work = block(wbytes);
+        work = carve(wbytes);
     }
-    if (!dA || !dB || !dC || (int8 && !work)) return UVT_ERR_NOMEM;
+    if (cursor > s->arena.at(sl.off) + sl.p.bytes) return UVT_ERR_NOMEM;  // cannot happen
     sl.dC = dC;
     hipStream_t st = sl.stream;
     uvt_status e = UVT_OK;
@@ -246,8 +243,8 @@ static void scheduler(uvt_server s)
                 }
             }
             const double t = s->now();
-            for (const Block& b : sl.blocks) s->arena.free(b.off, b.bytes);
-            sl.blocks.clear();
+            s->arena.free(sl.off, sl.p.bytes);
+            sl.off = SIZE_MAX;
             uvt_completion c;
             memset(&c, 0, sizeof c);
             c.ticket = sl.p.ticket;
@@ -278,8 +275,13 @@ static void scheduler(uvt_server s)
                 }
             if (!free_slot) break;
             const Pending& head = s->pending.front();
-            if (s->arena.used() + head.bytes > s->arena.size()) break;  // wait for memory
+            // one contiguous block per request; none free yet: wait for
+            // completions (with nothing in flight the arena is one free block,
+            // and every queued request fits the budget)
+            const size_t off = s->arena.alloc(head.bytes);
+            if (off == SIZE_MAX) break;
             free_slot->busy = true;
+            free_slot->off = off;
             free_slot->p = head;
             free_slot->t_admit = s->now();
             free_slot->status = UVT_OK;

@@ -92,6 +92,16 @@ struct Desc {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
+// Pointers read from a descriptor are generic; through these the compiler
+// emits global (not flat) loads: scalar base + 32-bit lane offset, and no
+// lgkmcnt coupling with the LDS traffic.
+#define UVT_GLOBAL __attribute__((address_space(1)))
+__device__ __forceinline__ v4i ld_global16(const int8_t* base, uint32_t off)
+{
+    const UVT_GLOBAL int8_t* g = (const UVT_GLOBAL int8_t*)base;  // address-space cast
+    return *(const UVT_GLOBAL v4i*)(g + off);
+}
+
 // The problem of this workgroup (a copy in scalar registers: every field is
 // uniform over the workgroup).
 __device__ __forceinline__ Desc fetch(const Desc& one, const Desc* table, const int* lut, int nprob)
@@ -260,34 +270,34 @@ __device__ __forceinline__ void i8_tile_loop(const int8_t* __restrict__ Af, int6
                                              int8_t* smem, v16i (&acc)[2][4])
 {
     constexpr int KS = kI8BK / 32;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // Addresses are a wave-uniform base (scalar registers) plus one 32-bit
+    // per-lane offset, which keeps the vector registers for the operands.
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave >> 1, wn = wave & 1;
     // this lane's 16 bytes of fragment (row block 2*wm + i, k block kb)
-    const int8_t* Aw = Af + (int64_t)(2 * wm) * kbn * kFrag + lane * 16;
+    const int8_t* Awave = Af + (int64_t)(2 * wm) * kbn * kFrag;
+    const uint32_t aoff = (uint32_t)lane * 16;
+    // B panel rows tid/8 + 64 q, 16 bytes at (tid%8)*16 of the stage
+    const uint32_t boff = (uint32_t)(tid >> 3) * (uint32_t)ldb_r + (uint32_t)(tid & 7) * 16;
+    const uint32_t soff = (uint32_t)(tid >> 3) * kI8S + (uint32_t)(tid & 7) * 16;
     v4i a0[KS][2], a1[KS][2], rb[4];
     auto load_a = [&](int k0, v4i (&d)[KS][2]) {
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
             for (int i = 0; i < 2; ++i)
-                d[ks][i] = *reinterpret_cast<const v4i*>(Aw + ((int64_t)i * kbn + (k0 >> 5) + ks) * kFrag);
+                d[ks][i] = ld_global16(Awave + ((int64_t)i * kbn + (k0 >> 5) + ks) * kFrag, aoff);
     };
     auto load_b = [&](int k0) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int c = tid + q * 512;
-            const int row = c >> 3, seg = (c & 7) * 16;
-            rb[q] = *reinterpret_cast<const v4i*>(B + row * ldb_r + k0 + seg);
-        }
+        for (int q = 0; q < 4; ++q)
+            rb[q] = ld_global16(B + (int64_t)q * 64 * ldb_r + k0, boff);
     };
     auto store_b = [&](int buf) {
-        int8_t* bs = smem + buf * kI8Stage;
+        int8_t* bs = smem + buf * kI8Stage + soff;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int c = tid + q * 512;
-            const int row = c >> 3, seg = (c & 7) * 16;
-            *reinterpret_cast<v4i*>(bs + row * kI8S + seg) = rb[q];
-        }
+        for (int q = 0; q < 4; ++q) *reinterpret_cast<v4i*>(bs + q * 64 * kI8S) = rb[q];
     };
     auto compute = [&](int buf, v4i (&a)[KS][2]) {
         const int8_t* bs = smem + buf * kI8Stage + (wn * 128 + (lane & 31)) * kI8S + (lane >> 5) * 16;
