@@ -25,8 +25,8 @@ void k_trmm_tiles_f64(const uvt_tile* __restrict__ tiles, const double* __restri
     using W = TileWaves<TM, TN>;
     __shared__ double smem[F64Cfg<TM, TN, W::M, W::N>::SMEM_DOUBLES];
     const uvt_tile t = tiles[blockIdx.x];
-    f64_tile<TM, TN, VEC, W::M, W::N>(t.r0, t.rows, t.c0, t.k_lo, t.k_hi, t.flags, A, lda, B, ldb,
-                                      C, ldc, m, smem);
+    f64_tile<TM, TN, VEC, W::M, W::N>(t.r0, t.rows, t.c0, t.k_lo, t.k_hi, t.flags, gbl(A), lda,
+                                      gbl(B), ldb, gbl(C), ldc, m, smem);
 }
 
 // The trapezoid walk without a tile list: workgroup b takes row tile
@@ -43,8 +43,8 @@ void k_trmm_trapezoid_f64(int64_t n, int64_t m, int nrt, int nct, const double* 
     const int rt = nrt - 1 - (int)(blockIdx.x / nct), ct = (int)(blockIdx.x % nct);
     const int r0 = rt * TM;
     const int rows = (int)(n - r0 < TM ? n - r0 : TM);
-    f64_tile<TM, TN, VEC, W::M, W::N>(r0, rows, ct * TN, 0, r0 + rows, kTileLower, A, lda, B, ldb, C,
-                                      ldc, m, smem);
+    f64_tile<TM, TN, VEC, W::M, W::N>(r0, rows, ct * TN, 0, r0 + rows, kTileLower, gbl(A), lda,
+                                      gbl(B), ldb, gbl(C), ldc, m, smem);
 }
 
 // One wave per row: sum_j |C[i][j]|, lanes strided over columns, then a

@@ -69,8 +69,9 @@ void k_batch_f64(const BatchTile* __restrict__ tiles, int ntiles, const BatchPro
         if (t >= ntiles) break;
         const BatchTile bt = tiles[t];
         const BatchProb p = probs[bt.prob];
-        f64_tile<TM, TN, VEC, W::M, W::N>(bt.r0, bt.rows, bt.c0, bt.k_lo, bt.k_hi, kTileLower, p.A,
-                                          p.lda, p.B, p.ldb, p.C, p.ldc, p.m, smem);
+        f64_tile<TM, TN, VEC, W::M, W::N>(bt.r0, bt.rows, bt.c0, bt.k_lo, bt.k_hi, kTileLower,
+                                          gbl(p.A), p.lda, gbl(p.B), p.ldb, gbl(p.C), p.ldc, p.m,
+                                          smem);
         __syncthreads();  // the LDS tile is reused by the next tile
     }
 }

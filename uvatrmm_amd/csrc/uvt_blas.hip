@@ -120,7 +120,8 @@ void k_gemm_f64(int64_t M, int64_t N, int K, int nct, const double* __restrict__
     const int rt = (int)(blockIdx.x / nct), ct = (int)(blockIdx.x % nct);
     const int r0 = rt * TM;
     const int rows = (int)(M - r0 < TM ? M - r0 : TM);
-    f64_tile<TM, TN, VEC, W::M, W::N>(r0, rows, ct * TN, 0, K, 0, A, lda, B, ldb, C, ldc, N, smem);
+    f64_tile<TM, TN, VEC, W::M, W::N>(r0, rows, ct * TN, 0, K, 0, gbl(A), lda, gbl(B), ldb, gbl(C),
+                                      ldc, N, smem);
 }
 
 template <int TM, int TN>

@@ -24,6 +24,23 @@ namespace uvt {
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef double dbl2 __attribute__((ext_vector_type(2)));
 
+// Global-memory pointers.  A generic pointer (one read from a descriptor in
+// memory, say) is accessed with flat instructions: 64-bit addresses, and
+// their completion is counted together with LDS traffic, so a wait for an
+// LDS read also waits for every load in flight.  Typed global, the same
+// access compiles to a global load or store (scalar base + lane offset).
+#define UVT_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const UVT_GLOBAL T* gbl(const T* p)
+{
+    return (const UVT_GLOBAL T*)p;
+}
+template <typename T>
+__device__ __forceinline__ UVT_GLOBAL T* gbl(T* p)
+{
+    return (UVT_GLOBAL T*)p;
+}
+
 constexpr int kTileLower = 1;  // UVT_TILE_LOWER
 constexpr int kTileAccum = 2;  // UVT_TILE_ACCUM
 
@@ -50,9 +67,9 @@ struct F64Cfg {
 // VEC: A, B 16-byte aligned and lda, ldb even, so 2-double loads are legal.
 template <int TM, int TN, bool VEC, int WAVES_M = 2, int WAVES_N = 2, int BK_ = 16>
 __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int k_hi, int flags,
-                                         const double* __restrict__ A, int64_t lda,
-                                         const double* __restrict__ B, int64_t ldb,
-                                         double* __restrict__ C, int64_t ldc, int64_t m,
+                                         const UVT_GLOBAL double* __restrict__ A, int64_t lda,
+                                         const UVT_GLOBAL double* __restrict__ B, int64_t ldb,
+                                         UVT_GLOBAL double* __restrict__ C, int64_t ldc, int64_t m,
                                          double* smem)
 {
     using G = F64Cfg<TM, TN, WAVES_M, WAVES_N, BK_>;
@@ -85,9 +102,9 @@ __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int
             const int v = tid + q * NT;
             const int row = v / (BK / 2), kp = (v % (BK / 2)) * 2;
             const int gk = k0 + kp, gr = r0 + row;
-            const double* p = A + (int64_t)gr * lda + gk;
+            const UVT_GLOBAL double* p = A + (int64_t)gr * lda + gk;
             if (a_full) {
-                if (VEC) ra[q] = *reinterpret_cast<const dbl2*>(p);
+                if (VEC) ra[q] = *reinterpret_cast<const UVT_GLOBAL dbl2*>(p);
                 else ra[q] = dbl2{p[0], p[1]};
             } else {
                 const bool rv = row < rows;
@@ -101,9 +118,9 @@ __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int
             const int v = tid + q * NT;
             const int kr = v / (TN / 2), col = (v % (TN / 2)) * 2;
             const int gk = k0 + kr;
-            const double* p = B + (int64_t)gk * ldb + (c0 + col);
+            const UVT_GLOBAL double* p = B + (int64_t)gk * ldb + (c0 + col);
             if (b_full) {
-                if (VEC) rb[q] = *reinterpret_cast<const dbl2*>(p);
+                if (VEC) rb[q] = *reinterpret_cast<const UVT_GLOBAL dbl2*>(p);
                 else rb[q] = dbl2{p[0], p[1]};
             } else {
                 const bool kv = gk >= k_lo && gk < k_hi;
@@ -178,7 +195,7 @@ __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int
             for (int v = 0; v < 4; ++v) {
                 const int row = wm * G::WM + i * 16 + (lane >> 4) + 4 * v;
                 if (cv && row < rows) {
-                    double* p = C + (int64_t)(r0 + row) * ldc + (c0 + col);
+                    UVT_GLOBAL double* p = C + (int64_t)(r0 + row) * ldc + (c0 + col);
                     if (accum) *p += acc[i][j][v];
                     else *p = acc[i][j][v];
                 }

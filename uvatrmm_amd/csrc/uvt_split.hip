@@ -92,10 +92,8 @@ struct Desc {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
-// Pointers read from a descriptor are generic; through these the compiler
-// emits global (not flat) loads: scalar base + 32-bit lane offset, and no
-// lgkmcnt coupling with the LDS traffic.
-#define UVT_GLOBAL __attribute__((address_space(1)))
+// 16 bytes at base + off as a global load (scalar base + 32-bit lane offset;
+// base comes from a descriptor, see gbl()).
 __device__ __forceinline__ v4i ld_global16(const int8_t* base, uint32_t off)
 {
     const UVT_GLOBAL int8_t* g = (const UVT_GLOBAL int8_t*)base;  // address-space cast
@@ -106,7 +104,15 @@ __device__ __forceinline__ v4i ld_global16(const int8_t* base, uint32_t off)
 // uniform over the workgroup).
 __device__ __forceinline__ Desc fetch(const Desc& one, const Desc* table, const int* lut, int nprob)
 {
-    return nprob == 1 ? one : table[lut[blockIdx.x]];
+    if (nprob == 1) return one;
+    // word by word through a global pointer (global, not flat, loads)
+    static_assert(sizeof(Desc) % 4 == 0, "Desc is copied in 4-byte words");
+    Desc d;
+    const UVT_GLOBAL int* src = (const UVT_GLOBAL int*)gbl(&table[gbl(lut)[blockIdx.x]]);
+    int* dst = reinterpret_cast<int*>(&d);
+#pragma unroll
+    for (int w = 0; w < (int)(sizeof(Desc) / 4); ++w) dst[w] = src[w];
+    return d;
 }
 
 __device__ __forceinline__ int64_t row_box_end(int64_t i) { return (i / kTileI) * kTileI; }
@@ -125,14 +131,14 @@ __global__ __launch_bounds__(256) void k_alpha(Desc one, const Desc* table, cons
     const int lane = threadIdx.x & 63;
     if (row >= d.n) return;
     const int64_t kb = row_box_end(row);
-    const double* a = d.A + row * d.lda;
+    const UVT_GLOBAL double* a = gbl(d.A) + row * d.lda;
     double s = 0.0;
     for (int64_t k = lane; k < kb; k += 64) s = fma(a[k], a[k], s);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     if (lane == 0) {
         const double nrm = sqrt(s);
-        d.alpha[row] = nrm > 0.0 ? ilogb(d.target / nrm) : 0;
+        gbl(d.alpha)[row] = nrm > 0.0 ? ilogb(d.target / nrm) : 0;
     }
 }
 
@@ -149,10 +155,10 @@ __global__ __launch_bounds__(256) void k_beta_part(Desc one, const Desc* table, 
     const int64_t k1 = k0 + 256 < d.kmax ? k0 + 256 : d.kmax;
     double s = 0.0;
     for (int64_t k = k0; k < k1; ++k) {
-        const double b = d.B[k * d.ldb + j];
+        const double b = gbl(d.B)[k * d.ldb + j];
         s = fma(b, b, s);
     }
-    d.part[chunk * d.m + j] = s;
+    gbl(d.part)[chunk * d.m + j] = s;
 }
 
 __global__ __launch_bounds__(256) void k_beta(Desc one, const Desc* table, const int* lut, int nprob)
@@ -162,9 +168,9 @@ __global__ __launch_bounds__(256) void k_beta(Desc one, const Desc* table, const
     const int64_t j = wb * 256 + threadIdx.x;
     if (j >= d.m) return;
     double s = 0.0;
-    for (int64_t c = 0; c < d.chunks; ++c) s += d.part[c * d.m + j];
+    for (int64_t c = 0; c < d.chunks; ++c) s += gbl(d.part)[c * d.m + j];
     const double nrm = sqrt(s);
-    d.beta[j] = nrm > 0.0 ? ilogb(d.target / nrm) : 0;
+    gbl(d.beta)[j] = nrm > 0.0 ? ilogb(d.target / nrm) : 0;
 }
 
 // ---- residues ---------------------------------------------------------------
@@ -199,13 +205,13 @@ __global__ __launch_bounds__(256) void k_resid_a(Desc one, const Desc* table, co
     const int64_t i = rb * 32 + (t >> 3);
     const int64_t k0 = kq * 128 + c * 16;
     const bool iv = i < d.n;
-    const int al = iv ? d.alpha[i] : 0;
+    const int al = iv ? gbl(d.alpha)[i] : 0;
     double v[16];
-    const double* a = d.A + i * d.lda + k0;
+    const UVT_GLOBAL double* a = gbl(d.A) + i * d.lda + k0;
 #pragma unroll
     for (int e = 0; e < 16; ++e) v[e] = iv ? rint(ldexp(a[e], al)) : 0.0;
     const int64_t kbn = d.kmax / 32, mod_stride = pad_tile(d.n) * d.kmax;
-    int8_t* out = d.Ar + ((rb * kbn + kq * 4 + (c >> 1)) * 64 + (t >> 3) + 32 * (c & 1)) * 16;
+    UVT_GLOBAL int8_t* out = gbl(d.Ar) + ((rb * kbn + kq * 4 + (c >> 1)) * 64 + (t >> 3) + 32 * (c & 1)) * 16;
     for (int l = 0; l < mods.n; ++l) {
         uint32_t q4[4];
 #pragma unroll
@@ -217,7 +223,7 @@ __global__ __launch_bounds__(256) void k_resid_a(Desc one, const Desc* table, co
                      << (8 * e);
             q4[w] = x;
         }
-        *reinterpret_cast<v4i*>(out + l * mod_stride) =
+        *reinterpret_cast<UVT_GLOBAL v4i*>(out + l * mod_stride) =
             v4i{(int)q4[0], (int)q4[1], (int)q4[2], (int)q4[3]};
     }
 }
@@ -235,12 +241,12 @@ __global__ __launch_bounds__(256) void k_resid_b(Desc one, const Desc* table, co
     const int jj = threadIdx.x & 63, kq = threadIdx.x >> 6;
     const int64_t j = j0 + jj;
     const bool jv = j < d.m;
-    const int be = jv ? d.beta[j] : 0;
+    const int be = jv ? gbl(d.beta)[j] : 0;
     double v[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int64_t k = k0 + kq + 4 * q;  // < kmax: kmax is a multiple of 64
-        v[q] = jv ? rint(ldexp(d.B[k * d.ldb + j], be)) : 0.0;
+        v[q] = jv ? rint(ldexp(gbl(d.B)[k * d.ldb + j], be)) : 0.0;
     }
     const int wr = threadIdx.x >> 2, ws = (threadIdx.x & 3) * 16;
     const int64_t mod_stride = pad_tile(d.m) * d.kmax;
@@ -251,7 +257,7 @@ __global__ __launch_bounds__(256) void k_resid_b(Desc one, const Desc* table, co
         __syncthreads();
         {
             const v4i x = *reinterpret_cast<const v4i*>(&t[wr][ws]);
-            *reinterpret_cast<v4i*>(d.Br + l * mod_stride + (j0 + wr) * d.kmax + k0 + ws) = x;
+            *reinterpret_cast<UVT_GLOBAL v4i*>(gbl(d.Br) + l * mod_stride + (j0 + wr) * d.kmax + k0 + ws) = x;
         }
         __syncthreads();
     }
@@ -366,7 +372,7 @@ __global__ __launch_bounds__(512, 1) void k_gemm_i8(Desc one, const Desc* table,
     const int64_t kbn = kmax / 32;
     const int8_t* Af = UVT_FIELD(Ar) + l * (pad_tile(n) * kmax) + (r0 / 32) * kbn * kFrag;
     const int8_t* B = UVT_FIELD(Br) + l * (pad_tile(m) * kmax) + c0 * kmax;
-    uint8_t* const Rout = UVT_FIELD(R);
+    UVT_GLOBAL uint8_t* const Rout = gbl(UVT_FIELD(R));
 #undef UVT_FIELD
 
     v16i acc[2][4];
@@ -382,7 +388,7 @@ __global__ __launch_bounds__(512, 1) void k_gemm_i8(Desc one, const Desc* table,
     const int wm = wave >> 1, wn = wave & 1;
     const int mod = mods.m[l];
     const float invf = mods.invf[l];
-    uint8_t* out = Rout + l * (n * m);
+    UVT_GLOBAL uint8_t* out = Rout + l * (n * m);
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -418,14 +424,15 @@ __global__ __launch_bounds__(512, 4) void k_diag_f64(Desc one, const Desc* table
     const int r0 = rt * 128, c0 = ct * 128;
     const int rows = (int)(n - r0 < 128 ? n - r0 : 128);
     const int k_lo = (r0 / kTileI) * kTileI;
+    const UVT_GLOBAL double* A = gbl(UVT_FIELD(A));
+    const UVT_GLOBAL double* B = gbl(UVT_FIELD(B));
+    UVT_GLOBAL double* C = gbl(UVT_FIELD(C));
     if (UVT_FIELD(vec))
-        f64_tile<128, 128, true, 4, 2>(r0, rows, c0, k_lo, r0 + rows, kTileLower, UVT_FIELD(A),
-                                       UVT_FIELD(lda), UVT_FIELD(B), UVT_FIELD(ldb), UVT_FIELD(C),
-                                       UVT_FIELD(ldc), UVT_FIELD(m), smem);
+        f64_tile<128, 128, true, 4, 2>(r0, rows, c0, k_lo, r0 + rows, kTileLower, A, UVT_FIELD(lda),
+                                       B, UVT_FIELD(ldb), C, UVT_FIELD(ldc), UVT_FIELD(m), smem);
     else
-        f64_tile<128, 128, false, 4, 2>(r0, rows, c0, k_lo, r0 + rows, kTileLower, UVT_FIELD(A),
-                                        UVT_FIELD(lda), UVT_FIELD(B), UVT_FIELD(ldb), UVT_FIELD(C),
-                                        UVT_FIELD(ldc), UVT_FIELD(m), smem);
+        f64_tile<128, 128, false, 4, 2>(r0, rows, c0, k_lo, r0 + rows, kTileLower, A, UVT_FIELD(lda),
+                                        B, UVT_FIELD(ldb), C, UVT_FIELD(ldc), UVT_FIELD(m), smem);
 #undef UVT_FIELD
 }
 
@@ -523,9 +530,9 @@ __global__ __launch_bounds__(256) void k_reconstruct(Desc one, const Desc* table
     uint32_t res[N][4];
 #pragma unroll
     for (int l = 0; l < N; ++l) {
-        const uint8_t* r = d.R + l * mod_stride + i * m + j4;
+        const UVT_GLOBAL uint8_t* r = gbl(d.R) + l * mod_stride + i * m + j4;
         if (vec4) {
-            const uint32_t x = *reinterpret_cast<const uint32_t*>(r);
+            const uint32_t x = *reinterpret_cast<const UVT_GLOBAL uint32_t*>(r);
 #pragma unroll
             for (int e = 0; e < 4; ++e) res[l][e] = (x >> (8 * e)) & 255u;
         } else {
@@ -533,7 +540,7 @@ __global__ __launch_bounds__(256) void k_reconstruct(Desc one, const Desc* table
             for (int e = 0; e < 4; ++e) res[l][e] = e < ne ? r[e] : 0u;
         }
     }
-    const int ai = d.alpha[i];
+    const int ai = gbl(d.alpha)[i];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         if (e >= ne) break;
@@ -558,7 +565,7 @@ __global__ __launch_bounds__(256) void k_reconstruct(Desc one, const Desc* table
         }
         const double x = fma(h, G.wlo_hi, fma(h, G.wlo_lo, low));
         const int64_t j = j4 + e;
-        d.C[i * d.ldc + j] += ldexp(x, -(ai + d.beta[j]));
+        gbl(d.C)[i * d.ldc + j] += ldexp(x, -(ai + gbl(d.beta)[j]));
     }
 }
 
