@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace uvt {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
@@ -93,10 +95,13 @@ __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int
 
     dbl2 ra[G::A_VEC], rb[G::B_VEC];
 
-    auto load = [&](int k0) {
-        const bool a_full = rows == TM && k0 >= k_lo && k0 + BK <= k_hi &&
-                            (!lower || k0 + BK - 1 <= r0);
-        const bool b_full = cols_full && k0 >= k_lo && k0 + BK <= k_hi;
+    // full_tag: std::true_type when the whole k panel lies inside the tile's
+    // k range, below the diagonal and inside C (no masks at all); otherwise
+    // every element is checked.
+    auto load = [&](int k0, auto full_tag) {
+        constexpr bool FULL = decltype(full_tag)::value;
+        const bool a_full = FULL;
+        const bool b_full = FULL;
 #pragma unroll
         for (int q = 0; q < G::A_VEC; ++q) {
             const int v = tid + q * NT;
@@ -168,14 +173,32 @@ __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int
         }
     };
 
+    // Panels kt in [kt0, nfull) need no masks: k0 >= k_lo (only panel 0 can
+    // start below an odd k_lo), k0 + BK <= k_hi, left of the diagonal for a
+    // lower tile, and the tile inside C.  They form the steady-state loop,
+    // free of per-panel branches; the rest (next to the diagonal, at the
+    // tile's k end) run masked.
     if (nk > 0) {
-        load(k_start);
+        const std::true_type full{};
+        const std::false_type masked{};
+        const int lim = lower ? (k_hi < r0 + 1 ? k_hi : r0 + 1) : k_hi;
+        const int kt0 = k_start < k_lo ? 1 : 0;
+        const int nfull = (rows == TM && cols_full && lim >= k_start) ? (lim - k_start) / BK : 0;
+        if (kt0 == 0 && nfull > 0) load(k_start, full);
+        else load(k_start, masked);
         store(0);
         __syncthreads();
-        for (int kt = 0; kt < nk; ++kt) {
+        int kt = 0;
+        for (; kt + 1 < nfull; ++kt) {  // the next panel is full
+            load(k_start + (kt + 1) * BK, full);
+            compute(kt & 1);
+            store((kt & 1) ^ 1);
+            __syncthreads();
+        }
+        for (; kt < nk; ++kt) {
             const int cur = kt & 1;
             const bool more = kt + 1 < nk;
-            if (more) load(k_start + (kt + 1) * BK);
+            if (more) load(k_start + (kt + 1) * BK, masked);
             compute(cur);
             if (more) store(cur ^ 1);
             __syncthreads();
