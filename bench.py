@@ -93,9 +93,10 @@ def cpu_baseline(args, C_gpu_cols, cols_needed):
 INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 MFMA peak
 
 
-def split_leg(args, h, uvt, dA, dB, dC, flops, plain_ms):
-    """Split-precision TRMM on the same inputs; error against the FP64 result."""
-    n, m, nm = args.n, args.m, args.split_moduli
+def split_leg(args, h, uvt, dA, dB, dC, flops, plain_ms, nm):
+    """Split-precision TRMM with nm moduli on the same inputs; error against
+    the FP64 result."""
+    n, m = args.n, args.m
     ws = h.split_workspace(n, m, nm)
     log(f"split: {nm} moduli, workspace {ws / 2**30:.1f} GiB")
     dW, dS = h.alloc(ws), h.alloc(n * m * 8)
@@ -272,8 +273,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-cols-per-rank", type=int, default=256)
     ap.add_argument("--batch", type=int, default=512, help="problems of the batch leg (0 = skip)")
-    ap.add_argument("--split-moduli", type=int, default=13,
-                    help="moduli of the split-precision leg (0 = skip it)")
+    ap.add_argument("--split-moduli", type=int, default=12,
+                    help="moduli of the split-precision leg (0 = skip it); 12 keeps the "
+                         "error below 1e-12 at N=32768")
+    ap.add_argument("--split-moduli-alt", type=int, nargs="*", default=[13],
+                    help="further moduli counts measured beside it (time and error)")
     ap.add_argument("--serve-requests", type=int, default=2000,
                     help="measured requests of the served stream (0 = skip)")
     ap.add_argument("--serve-warmup", type=int, default=100)
@@ -318,9 +322,12 @@ def main():
         f"wall/step {wall / args.steps * 1e3:.2f} ms; {achieved:.2f} TFLOP/s")
     norm = h.inf_norm(n, m, dC, m)
 
-    split = None
+    split, split_alt = None, []
     if args.split_moduli > 0:
-        split = split_leg(args, h, uvt, dA, dB, dC, flops, mean_ms)
+        split = split_leg(args, h, uvt, dA, dB, dC, flops, mean_ms, args.split_moduli)
+        for nm in args.split_moduli_alt:
+            if nm != args.split_moduli:
+                split_alt.append(split_leg(args, h, uvt, dA, dB, dC, flops, mean_ms, nm))
 
     batch = batch_leg(args, h, uvt) if args.batch > 0 else None
 
@@ -368,6 +375,7 @@ def main():
                      "algorithmic_flops_per_step": flops},
         "cpu_baseline": cpu,
         "split": split,
+        "split_other_moduli": split_alt,
         "batch": batch,
         "serve": serve,
         "norm": norm,

@@ -103,3 +103,13 @@ print("ok")
     assert [l.split("parameter ")[1].split()[0] for l in lines] == \
         ["4", "1", "11", "9", "10", "5", "12"], lines
     assert all("cblas_dgemm" in l for l in lines[:4]) and all("cblas_dtrmm" in l for l in lines[4:])
+
+
+def test_split_refuses_too_few_moduli(uvt):
+    """Two moduli cannot carry a reduction of 130000 (the scaled operands
+    would keep no bits): the workspace query refuses the size instead of
+    letting the product come out wrong.  Host-only check."""
+    b = ctypes.c_size_t()
+    assert uvt.lib().uvt_trmm_split_workspace(130000, 64, 2, ctypes.byref(b)) == uvt.ERR_SIZE
+    assert uvt.lib().uvt_trmm_split_workspace(2000, 64, 2, ctypes.byref(b)) == uvt.OK
+    assert uvt.lib().uvt_trmm_split_workspace(130000, 64, 12, ctypes.byref(b)) == uvt.OK

@@ -82,3 +82,30 @@ def test_workspace_errors(gpu, uvt):
         gpu.split_workspace(100, 100, 1)
     with pytest.raises(uvt.UvtError):
         gpu.split_workspace(100, 100, 17)
+
+
+def test_twelve_moduli_large_k(gpu, uvt, oracle):
+    """12 moduli at a deep reduction (n = 16384, exponents over 2^[-8, 8]) stay
+    within the 1e-12 bar; the reference is the FP64 path (itself within
+    1e-13 of the oracle), here computed on the GPU."""
+    n, m = 16384, 96
+    say("12 moduli", n, m)
+    dA, dB = gpu.alloc(n * n * 8), gpu.alloc(n * m * 8)
+    dC, dS = gpu.alloc(n * m * 8), gpu.alloc(n * m * 8)
+    ws = gpu.split_workspace(n, m, 12)
+    dW = gpu.alloc(ws)
+    try:
+        gpu.fill_seeded(n, m, 77, 1, dA, n, dB, m)
+        plan = uvt.Plan(n, m, 8, "balanced", "trapezoid")
+        gpu.trmm_f64(plan, dA, n, dB, m, dC, m)
+        gpu.trmm_split(n, m, 12, dA, n, dB, m, dS, m, dW)
+        gpu.sync()
+        ref, got = dC.download((n, m)), dS.download((n, m))
+        plan.close()
+    finally:
+        for b in (dA, dB, dC, dS, dW):
+            b.free()
+    err = rel_fro(got, ref)
+    say("rel_fro 12 moduli", err)
+    assert err <= TOL
+

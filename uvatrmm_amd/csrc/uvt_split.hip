@@ -556,7 +556,10 @@ __global__ __launch_bounds__(256) void k_reconstruct(Desc one, const Desc* table
         const uint64_t xlo = digits_value<0, GarnerTab<N>::split>(v);
         const uint64_t xhi = digits_value<GarnerTab<N>::split, N>(v);
         double h, low;
-        if (2 * xhi > G.qhi) {  // x > M/2: value x - M = (X_hi - Q + 1) W + (X_lo - W)
+        // x > M/2 exactly:  2x - M = W (2 X_hi - Q) + 2 X_lo with 0 <= X_lo < W
+        const int64_t dq = 2 * (int64_t)xhi - (int64_t)G.qhi;
+        const bool neg = dq >= 1 || (dq == 0 && xlo > 0) || (dq == -1 && 2 * xlo > G.wlo);
+        if (neg) {  // value x - M = (X_hi - Q + 1) W + (X_lo - W)
             h = (double)((int64_t)xhi - (int64_t)G.qhi + 1);
             low = (double)((int64_t)xlo - (int64_t)G.wlo);
         } else {
@@ -580,20 +583,38 @@ void launch_reconstruct(unsigned grid, hipStream_t s, const Desc& one, const Des
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-Mods make_mods(int nmod, double* log2M)
+Mods make_mods(int nmod)
 {
     Mods md;
     memset(&md, 0, sizeof md);
     md.n = nmod;
-    double lg = 0.0;
     for (int l = 0; l < nmod; ++l) {
         md.m[l] = kModuli[l];
         md.invf[l] = 1.0f / (float)kModuli[l];
         md.invd[l] = 1.0 / (double)kModuli[l];
-        lg += log2((double)kModuli[l]);
     }
-    *log2M = lg;
     return md;
+}
+
+// Norm every scaled row of A' and column of B' stays below (before rounding
+// to integers; rounding k entries adds at most sqrt(k)/2 to a 2-norm):
+// |A'_i B'_j| <= |A'_i| |B'_j| <= 2^(2 lt) < M/2, so the residues determine
+// the signed product uniquely, and every entry stays below 2^52.
+double split_target(int64_t kmax, int nmod)
+{
+    double log2M = 0.0;
+    for (int l = 0; l < nmod; ++l) log2M += log2((double)kModuli[l]);
+    double lt = (log2M - 1.0) / 2.0 - 1e-6;  // margin for the rounding of log2M
+    if (lt > 52.0) lt = 52.0;
+    return exp2(lt) - 0.5 * sqrt((double)(kmax + 1));
+}
+
+// Fewer moduli than this problem needs: the scaled operands would keep
+// fewer than 6 bits.
+bool too_few_moduli(int64_t n, int nmod)
+{
+    const int64_t kmax = ((n - 1) / kTileI) * kTileI;
+    return kmax > 0 && split_target(kmax, nmod) < 64.0;
 }
 
 // Row tiles per XCD group of the int8 products.  8 x 4 groups share panels
@@ -642,10 +663,6 @@ uvt_status check_problem(const uvt_split_problem& p)
 void describe(const uvt_split_problem* probs, int count, int nmod, char* work, Desc* out,
               int64_t totals[K_NUM])
 {
-    double log2M = 0.0;
-    make_mods(nmod, &log2M);
-    double lt = (log2M - 2.0) / 2.0;  // |A'_i| |B'_j| <= M/4 and every scaled entry < 2^53
-    if (lt > 52.0) lt = 52.0;
     for (int k = 0; k < K_NUM; ++k) totals[k] = 0;
     size_t off = 0;
     for (int p = 0; p < count; ++p) {
@@ -661,7 +678,7 @@ void describe(const uvt_split_problem* probs, int count, int nmod, char* work, D
         d.n = q.n;
         d.m = q.m;
         d.kmax = ((q.n - 1) / kTileI) * kTileI;
-        d.target = ldexp(1.0, (int)floor(lt)) - 0.5 * sqrt((double)(d.kmax + 1));
+        d.target = split_target(d.kmax, nmod);
         d.nrt = (int32_t)((q.n + kTileI - 1) / kTileI);
         d.nct = (int32_t)((q.m + kTileI - 1) / kTileI);
         d.nrt128 = (int32_t)((q.n + 127) / 128);
@@ -720,6 +737,7 @@ uvt_status uvt_split_batch_sizes(const uvt_split_problem* probs, int count, int 
     for (int p = 0; p < count; ++p) {
         uvt_status st = check_problem(probs[p]);
         if (st != UVT_OK) return st;
+        if (too_few_moduli(probs[p].n, nmod)) return UVT_ERR_SIZE;
         w += problem_bytes(probs[p].n, probs[p].m, nmod);
     }
     int64_t lut = 0;
@@ -759,8 +777,7 @@ uvt_status uvt_split_batch_launch(const uvt_split_problem* single, int count, in
                                   const int64_t totals_in[8], const int64_t lut_off[8],
                                   hipStream_t s, int stages)
 {
-    double log2M = 0.0;
-    const Mods mods = make_mods(nmod, &log2M);
+    const Mods mods = make_mods(nmod);
     Desc one;
     memset(&one, 0, sizeof one);
     int64_t totals[K_NUM];
@@ -817,6 +834,7 @@ extern "C" uvt_status uvt_trmm_split_workspace(int64_t n, int64_t m, int32_t sli
 {
     if (!bytes || n <= 0 || m <= 0 || slices < 2 || slices > kMaxSlices) return UVT_ERR_ARG;
     if (n > 131072 || m > 0x3FFFFFFF) return UVT_ERR_SIZE;  // int32 accumulation bound on k
+    if (too_few_moduli(n, slices)) return UVT_ERR_SIZE;
     *bytes = problem_bytes(n, m, slices);
     if (*bytes == 0) *bytes = 256;
     return UVT_OK;
