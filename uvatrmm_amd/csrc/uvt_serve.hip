@@ -29,72 +29,40 @@
 
 #include "uvt_internal.h"
 #include "uvt_runtime.hpp"
+#include "uvt_serve_arena.hpp"
 
 namespace {
 
-size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+size_t align_up(size_t x) { return uvt::OffsetArena::granule(x); }
 
-// First-fit allocator over one device allocation, 256-byte granules.
+// The serving arena: one device allocation of the budget, carved by offsets.
 class Arena {
 public:
     bool init(size_t bytes)
     {
-        size_ = align_up(bytes);
-        if (hipMalloc(&base_, size_) != hipSuccess) {
+        offsets_.reset(bytes);
+        if (hipMalloc(&base_, offsets_.size()) != hipSuccess) {
             (void)hipGetLastError();
             base_ = nullptr;
             return false;
         }
-        free_[0] = size_;
         return true;
     }
     void release()
     {
         if (base_) (void)hipFree(base_);
         base_ = nullptr;
-        free_.clear();
+        offsets_.reset(0);
     }
-    // offset of a block of `bytes`, or SIZE_MAX
-    size_t alloc(size_t bytes)
-    {
-        bytes = align_up(bytes);
-        for (auto it = free_.begin(); it != free_.end(); ++it) {
-            if (it->second >= bytes) {
-                const size_t off = it->first, rest = it->second - bytes;
-                free_.erase(it);
-                if (rest) free_[off + bytes] = rest;
-                used_ += bytes;
-                return off;
-            }
-        }
-        return SIZE_MAX;
-    }
-    void free(size_t off, size_t bytes)
-    {
-        bytes = align_up(bytes);
-        used_ -= bytes;
-        auto next = free_.lower_bound(off);
-        if (next != free_.end() && off + bytes == next->first) {
-            bytes += next->second;
-            next = free_.erase(next);
-        }
-        if (next != free_.begin()) {
-            auto prev = std::prev(next);
-            if (prev->first + prev->second == off) {
-                prev->second += bytes;
-                return;
-            }
-        }
-        free_[off] = bytes;
-    }
+    size_t alloc(size_t bytes) { return offsets_.alloc(bytes); }
+    void free(size_t off, size_t bytes) { offsets_.free(off, bytes); }
     char* at(size_t off) const { return (char*)base_ + off; }
-    size_t used() const { return used_; }
-    size_t size() const { return size_; }
+    size_t used() const { return offsets_.used(); }
+    size_t size() const { return offsets_.size(); }
 
 private:
     void* base_ = nullptr;
-    size_t size_ = 0, used_ = 0;
-    std::map<size_t, size_t> free_;
+    uvt::OffsetArena offsets_;
 };
 
 struct Pending {
