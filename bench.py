@@ -131,7 +131,10 @@ def split_leg(args, h, uvt, dA, dB, dC, flops, plain_ms, nm):
 
 def batch_leg(args, h, uvt):
     """512 uneven TRMMs (orders log-uniform in [64, 4096], B columns uniform in
-    [16, 1024], alternating FP64 / split) in one hipGraph launch."""
+    [16, 1024], half FP64 / half split class) in one hipGraph launch.  At
+    these orders a split-class problem runs in FP64 (faster in a batch below
+    n = 8192, and at least as accurate); int8_path_share_of_flops says how
+    much of the work the int8 path took."""
     rng = np.random.default_rng(args.seed + 17)
     ns = np.rint(np.exp(rng.uniform(np.log(64), np.log(4096), args.batch))).astype(int)
     ms = rng.integers(16, 1025, args.batch)
@@ -171,7 +174,8 @@ def batch_leg(args, h, uvt):
         f"launch, {info['flops'] / (ms_step * 1e-3) * 1e-12:.2f} TFLOP/s-equivalent")
     return {"problems": args.batch, "value": round(info["flops"] / (ms_step * 1e-3) * 1e-12, 4),
             "unit": "TFLOP/s (sum m n^2 / launch time)", "ms_per_launch": round(ms_step, 4),
-            "flops": info["flops"], "fp64_share_of_flops": round(f64_flops / info["flops"], 4),
+            "flops": info["flops"], "fp64_class_share_of_flops": round(f64_flops / info["flops"], 4),
+            "int8_path_share_of_flops": round(info["int8_flops"] / info["flops"], 4),
             "fp64_tiles": info["fp64_tiles"], "graph_nodes": info["graph_nodes"],
             "graph_build_s": round(build_s, 3), "tuning_table": bool(table)}
 

@@ -191,8 +191,12 @@ uvt_status uvt_batch_run(uvt_batch b, void *stream);
 /* New pointers for the same shapes and precisions (descriptors live in
  * device memory, so the captured graph picks them up); synchronous. */
 uvt_status uvt_batch_update(uvt_batch b, const uvt_problem *probs, int32_t count);
-/* Algorithmic flops (sum of m n^2), FP64 tiles queued, graph nodes. */
-uvt_status uvt_batch_info(uvt_batch b, double *flops, int64_t *fp64_tiles, int32_t *graph_nodes);
+/* Algorithmic flops (sum of m n^2), the part of them whose box runs on the
+ * int8 path (split-class problems large enough for it to be faster than FP64
+ * in a batch: n >= 8192 and m >= 256; the others run in FP64), FP64 tiles
+ * queued, graph nodes.  Any pointer may be NULL. */
+uvt_status uvt_batch_info(uvt_batch b, double *flops, double *int8_flops, int64_t *fp64_tiles,
+                          int32_t *graph_nodes);
 uvt_status uvt_batch_destroy(uvt_batch b);
 
 /* FP64 tile shapes selectable per problem. */

@@ -31,8 +31,11 @@ def make_problems(uvt, gpu, oracle, sizes, seed=0):
     return probs, host, bufs
 
 
+# Odd positions are split class.  Below n = 8192 a split-class problem runs in
+# FP64 in a batch; (8200, 260) and (8448, 300) take the int8 path.
 SIZES = [(64, 16), (100, 33), (257, 64), (300, 1000), (513, 128), (700, 250), (1024, 512),
-         (1500, 48), (2049, 300), (4096, 96), (80, 700), (1200, 1024), (130, 17), (600, 600)]
+         (1500, 48), (2049, 300), (4096, 96), (80, 700), (1200, 1024), (130, 17), (600, 600),
+         (70, 30), (8200, 260), (90, 20), (8448, 300)]
 
 
 def test_batch_matches_alone_and_oracle(uvt, gpu, oracle):
@@ -42,6 +45,7 @@ def test_batch_matches_alone_and_oracle(uvt, gpu, oracle):
     info = batch.info()
     say("info", info)
     assert info["flops"] == sum(float(m) * n * n for n, m in SIZES)
+    assert info["int8_flops"] == 260.0 * 8200 * 8200 + 300.0 * 8448 * 8448
     batch.run()
     gpu.sync()
     results = [bufs[k][2].download((n, m)) for k, (n, m) in enumerate(SIZES)]

@@ -122,8 +122,8 @@ def test_serve_host_arrays(uvt, gpu, oracle):
 def test_serve_device_operands(uvt, gpu, oracle):
     """Operands already resident in HBM are read and written in place and the
     arena is charged only for the split workspace."""
-    cases = [(1100, 96, 1), (500, 40, 0), (2048, 256, 0)]
-    srv = uvt.Server(gpu, 1 << 30, max_slots=3)
+    cases = [(1100, 96, 1), (500, 40, 0), (2048, 256, 0), (2048, 300, 1)]  # last: int8 path
+    srv = uvt.Server(gpu, 1 << 30, max_slots=4)
     bufs = []
     try:
         for k, (n, m, prec) in enumerate(cases):
@@ -138,7 +138,7 @@ def test_serve_device_operands(uvt, gpu, oracle):
         srv.close()
     ws = max(uvt.serve_request_bytes(n, m, p) - uvt.serve_request_bytes(n, m, uvt.PREC_F64)
              for n, m, p in cases)
-    assert st["peak_bytes"] <= 3 * ws
+    assert 0 < st["peak_bytes"] <= len(cases) * ws
     for A, B, dC, prec in bufs:
         n, m = B.shape
         err = rel_fro(dC.download((n, m)), oracle.trmm(A, B))

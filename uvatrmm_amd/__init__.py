@@ -100,7 +100,8 @@ _SIGS = {
     "uvt_batch_create": (i32, [vp, _c.c_void_p, i32, i32, _c.c_char_p, _c.POINTER(vp)]),
     "uvt_batch_run": (i32, [vp, vp]),
     "uvt_batch_update": (i32, [vp, _c.c_void_p, i32]),
-    "uvt_batch_info": (i32, [vp, _c.POINTER(f64), _c.POINTER(i64), _c.POINTER(i32)]),
+    "uvt_batch_info": (i32, [vp, _c.POINTER(f64), _c.POINTER(f64), _c.POINTER(i64),
+                             _c.POINTER(i32)]),
     "uvt_batch_destroy": (i32, [vp]),
     "uvt_autotune_f64": (i32, [vp, _c.c_char_p]),
     "uvt_tune_choice": (i32, [_c.c_char_p, i64, i64, _c.POINTER(i32)]),
@@ -367,9 +368,11 @@ class Batch:
         self._arr = arr
 
     def info(self):
-        fl, tiles, nodes = f64(), i64(), i32()
-        check(lib().uvt_batch_info(self.ptr, _c.byref(fl), _c.byref(tiles), _c.byref(nodes)))
-        return {"flops": fl.value, "fp64_tiles": tiles.value, "graph_nodes": nodes.value}
+        fl, i8, tiles, nodes = f64(), f64(), i64(), i32()
+        check(lib().uvt_batch_info(self.ptr, _c.byref(fl), _c.byref(i8), _c.byref(tiles),
+                                   _c.byref(nodes)))
+        return {"flops": fl.value, "int8_flops": i8.value, "fp64_tiles": tiles.value,
+                "graph_nodes": nodes.value}
 
     def close(self):
         if self.ptr:

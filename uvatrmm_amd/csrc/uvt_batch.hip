@@ -219,7 +219,6 @@ void sort_longest_first(std::vector<BatchTile>& v)
 
 // Split-class problems whose integer box part is too small to pay for its
 // conversions run in FP64 (which meets the split tolerance).
-bool split_pays_off(int64_t n, int64_t m) { return n >= 768 && m >= 64; }
 
 }  // namespace
 
@@ -249,7 +248,7 @@ struct uvt_batch_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     size_t nodes = 0;
-    double flops = 0.0;
+    double flops = 0.0, int8_flops = 0.0;
 };
 
 static void batch_free(uvt_batch b)
@@ -339,9 +338,10 @@ extern "C" uvt_status uvt_batch_create(uvt_handle h, const uvt_problem* probs, i
         b->f64_index.push_back(slot);
         b->f64_probs.push_back(f64_desc(p));
         const int cfg = table.empty() ? uvt_default_cfg(p.n, p.m) : uvt_tune_pick(table, p.n, p.m);
-        const bool use_int8 = p.prec == UVT_PREC_SPLIT && split_pays_off(p.n, p.m);
+        const bool use_int8 = p.prec == UVT_PREC_SPLIT && uvt_split_faster_in_batch(p.n, p.m);
         trapezoid_tiles(slot, p.n, p.m, cfg, b->tiles[cfg * 2 + b->f64_probs.back().vec], use_int8);
         b->int8.push_back(use_int8);
+        if (use_int8) b->int8_flops += (double)p.m * (double)p.n * (double)p.n;
         if (use_int8) b->sp.push_back(uvt_split_problem{p.A, p.lda, p.B, p.ldb, p.C, p.ldc, p.n, p.m});
     }
     for (auto& v : b->tiles) sort_longest_first(v);
@@ -474,11 +474,12 @@ extern "C" uvt_status uvt_batch_update(uvt_batch b, const uvt_problem* probs, in
     return upload_split(b);
 }
 
-extern "C" uvt_status uvt_batch_info(uvt_batch b, double* flops, int64_t* fp64_tiles,
-                                     int32_t* graph_nodes)
+extern "C" uvt_status uvt_batch_info(uvt_batch b, double* flops, double* int8_flops,
+                                     int64_t* fp64_tiles, int32_t* graph_nodes)
 {
     if (!b) return UVT_ERR_ARG;
     if (flops) *flops = b->flops;
+    if (int8_flops) *int8_flops = b->int8_flops;
     if (fp64_tiles) {
         int64_t t = 0;
         for (auto& v : b->tiles) t += (int64_t)v.size();

@@ -52,3 +52,16 @@ struct uvt_tune_entry {
 bool uvt_tune_load(const char* path, std::vector<uvt_tune_entry>& out);
 // Shape for (n, m); the built-in default when no entry matches.
 int uvt_tune_pick(const std::vector<uvt_tune_entry>& table, int64_t n, int64_t m);
+
+// Where a split-class problem (error <= 1e-12 acceptable) runs on the int8
+// path rather than in FP64, which is always at least as accurate.  Measured on
+// MI355X with 12 moduli (split time / FP64 time):
+//   alone (serving): the int8 path is faster from m >= 256 and n >= 2048 while
+//     m <= 1024, and for every m >= 256 from n >= 6144 (1.2x - 1.7x there);
+//     narrow B (m < 256) never pays for the residue and reconstruction passes;
+//   in a batch (throughput, persistent FP64 queues): FP64 wins below n = 8192.
+inline bool uvt_split_faster_alone(int64_t n, int64_t m)
+{
+    return m >= 256 && (n >= 6144 || (n >= 2048 && m <= 1024));
+}
+inline bool uvt_split_faster_in_batch(int64_t n, int64_t m) { return n >= 8192 && m >= 256; }

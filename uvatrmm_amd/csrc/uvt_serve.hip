@@ -83,7 +83,6 @@ struct Slot {
     uvt_status status = UVT_OK;
 };
 
-bool split_pays_off(int64_t n, int64_t m) { return n >= 768 && m >= 64; }
 
 }  // namespace
 
@@ -121,7 +120,7 @@ static size_t request_bytes(const uvt_serve_config& cfg, int64_t n, int64_t m, i
     size_t b = 0;
     if (ab_in_arena) b += align_up((size_t)n * n * 8) + align_up((size_t)n * m * 8);
     if (c_in_arena) b += align_up((size_t)n * m * 8);
-    if (prec == UVT_PREC_SPLIT && split_pays_off(n, m)) {
+    if (prec == UVT_PREC_SPLIT && uvt_split_faster_alone(n, m)) {
         size_t w = 0;
         if (uvt_trmm_split_workspace(n, m, cfg.slices, &w) == UVT_OK) b += align_up(w);
     }
@@ -156,7 +155,7 @@ static uvt_status launch(uvt_server s, Slot& sl)
     const double* dA = own_ab ? (const double*)carve((size_t)n * n * 8) : r.A;
     const double* dB = own_ab ? (const double*)carve((size_t)n * m * 8) : r.B;
     double* dC = c_in_arena(r) ? (double*)carve((size_t)n * m * 8) : r.C;
-    const bool int8 = r.prec == UVT_PREC_SPLIT && split_pays_off(n, m);
+    const bool int8 = r.prec == UVT_PREC_SPLIT && uvt_split_faster_alone(n, m);
     size_t wbytes = 0;
     void* work = nullptr;
     if (int8) {
