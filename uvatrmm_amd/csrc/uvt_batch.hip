@@ -512,8 +512,9 @@ extern "C" uvt_status uvt_tune_choice(const char* tune_table, int64_t n, int64_t
 // ---- autotuner -----------------------------------------------------------------
 
 // For every (n, m) bucket, a batch of copies of one problem at the bucket's
-// upper edge is timed with each tile shape (persistent kernel, best of three
-// runs); the fastest shape is the bucket's entry.
+// upper edge (enough to fill the GPU, as a mixed batch does) is timed with
+// each tile shape (persistent kernel, best of three runs); the fastest shape
+// is the bucket's entry.
 extern "C" uvt_status uvt_autotune_f64(uvt_handle h, const char* out_path)
 {
     if (!h || !out_path) return UVT_ERR_ARG;
@@ -527,7 +528,10 @@ extern "C" uvt_status uvt_autotune_f64(uvt_handle h, const char* out_path)
     BatchTile* dtiles = nullptr;
     hipStream_t s = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    const int max_copies = 64;
+    // Enough copies of a bucket's problem to keep every CU busy the way a
+    // mixed batch does (~1.2e11 flops), within a 1 GiB C buffer.
+    const int max_copies = 4096;
+    const size_t c_bytes = (size_t)1 << 30;
     uvt_status st = UVT_OK;
     std::string json = "{\n  \"version\": 1,\n  \"device\": \"gfx950\",\n  \"what\": \"FP64 tile shape per (n, m) bucket, chosen by timing\",\n  \"table\": [\n";
     auto fail = [&](uvt_status e) {
@@ -536,7 +540,7 @@ extern "C" uvt_status uvt_autotune_f64(uvt_handle h, const char* out_path)
     };
     if (hipMalloc(&A, (size_t)nmax * nmax * 8) != hipSuccess ||
         hipMalloc(&B, (size_t)nmax * mmax * 8) != hipSuccess ||
-        hipMalloc(&C, (size_t)max_copies * nmax * mmax * 8 / 16 + (size_t)nmax * mmax * 8) != hipSuccess ||
+        hipMalloc(&C, c_bytes) != hipSuccess ||
         hipMalloc(&counter, 64) != hipSuccess || hipMalloc(&dprobs, max_copies * sizeof(BatchProb)) != hipSuccess ||
         hipStreamCreate(&s) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
         hipEventCreate(&e1) != hipSuccess) {
@@ -548,9 +552,7 @@ extern "C" uvt_status uvt_autotune_f64(uvt_handle h, const char* out_path)
     for (int64_t n : ns) {
         for (int64_t m : ms) {
             if (st != UVT_OK) break;
-            // enough copies for ~1e10 flops, C of every copy inside the buffer
-            int copies = (int)std::min<double>(max_copies, std::max(1.0, 1e10 / ((double)m * n * n)));
-            const size_t c_bytes = (size_t)max_copies * nmax * mmax * 8 / 16 + (size_t)nmax * mmax * 8;
+            int copies = (int)std::min<double>(max_copies, std::max(1.0, 1.2e11 / ((double)m * n * n)));
             while (copies > 1 && (size_t)copies * n * m * 8 > c_bytes) --copies;
             std::vector<BatchProb> hp(copies);
             for (int c = 0; c < copies; ++c) {
