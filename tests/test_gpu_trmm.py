@@ -257,3 +257,18 @@ def test_cli_result_dump(uvt, oracle):
         assert rel_fro(data, ref[:, begin:begin + size]) <= 1e-14
         begin += size
     assert begin == m
+
+
+def test_cli_split_precision_norm(uvt):
+    """uvatrmm.x with --prec split (default 13 moduli) on the reference's
+    closed-form inputs prints the reference's Norm line (the split path's
+    ~1e-13 relative error is far below the printed 3 decimals)."""
+    import tempfile
+    cwd = tempfile.mkdtemp()
+    for args, want in ((["2048", "reg", "boxes", "--bands", "4"], "61.447"),
+                       (["4096", "balanced", "trapezoid", "--bands", "8", "--slices", "12"], "243.282")):
+        say("cli split", args)
+        out = subprocess.run([uvt.CLI_PATH, *args, "--prec", "split"], capture_output=True,
+                             text=True, timeout=120, cwd=cwd)
+        assert out.returncode == 0, out.stderr
+        assert f"\nNorm: {want}\n" in out.stdout, out.stdout
