@@ -263,6 +263,36 @@ def serve_leg(args, h, uvt):
     return out
 
 
+def configs_leg(args, h, uvt):
+    """The reference's other configurations, FP64: N=2048 (M=N, reg, boxes,
+    P=4) and N=16384 (M=N, balanced, trapezoid, P=16), seeded inputs."""
+    out = []
+    for n, p, mp, shape in ((2048, 4, "reg", "boxes"), (16384, 16, "balanced", "trapezoid")):
+        m = n
+        dA, dB, dC = h.alloc(n * n * 8), h.alloc(n * m * 8), h.alloc(n * m * 8)
+        h.fill_seeded(n, m, args.seed, args.spread, dA, n, dB, m)
+        plan = uvt.Plan(n, m, p, mp, shape)
+        for _ in range(max(1, args.warmup)):
+            h.trmm_f64(plan, dA, n, dB, m, dC, m)
+        h.sync()
+        evs = [(h.event(), h.event()) for _ in range(args.steps)]
+        for e0, e1 in evs:
+            h.record(e0)
+            h.trmm_f64(plan, dA, n, dB, m, dC, m)
+            h.record(e1)
+        h.sync()
+        ms = float(np.mean([h.elapsed_ms(e0, e1) for e0, e1 in evs]))
+        tf = plan.flops / (ms * 1e-3) * 1e-12
+        log(f"config N={n} {mp} {shape} P={p}: {ms:.3f} ms, {tf:.2f} TFLOP/s")
+        out.append({"workload": f"trmm_f64 N=M={n} {mp} {shape} P={p}", "ms": round(ms, 4),
+                    "tflops": round(tf, 4), "frac": round(tf / FP64_PEAK_TFLOPS, 4),
+                    "launches": plan.phases})
+        plan.close()
+        for b in (dA, dB, dC):
+            b.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,6 +313,8 @@ def main():
                          "error below 1e-12 at N=32768")
     ap.add_argument("--split-moduli-alt", type=int, nargs="*", default=[13],
                     help="further moduli counts measured beside it (time and error)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the N=2048 and N=16384 reference configurations")
     ap.add_argument("--serve-requests", type=int, default=2000,
                     help="measured requests of the served stream (0 = skip)")
     ap.add_argument("--serve-warmup", type=int, default=100)
@@ -336,6 +368,8 @@ def main():
 
     batch = batch_leg(args, h, uvt) if args.batch > 0 else None
 
+    other = configs_leg(args, h, uvt) if not args.no_configs else None
+
     serve = serve_leg(args, h, uvt) if args.serve_requests > 0 else None
 
     cpu = None
@@ -382,6 +416,7 @@ def main():
         "split": split,
         "split_other_moduli": split_alt,
         "batch": batch,
+        "other_configs": other,
         "serve": serve,
         "norm": norm,
     }
