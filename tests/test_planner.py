@@ -105,7 +105,8 @@ def test_boxes_geometry_follows_the_reference(uvt):
     for s, (b, h) in enumerate(bands):
         r = rect[rect["band"] == s]
         assert set(r["k_hi"]) == {b + 1} and set(r["k_lo"]) == {0}
-        assert r["rows"].sum() == h * ((m + 127) // 128)
+        tn = plan.tile_shape()[1]
+        assert r["rows"].sum() == h * ((m + tn - 1) // tn)
         t = tri[tri["band"] == s]
         assert set(t["k_lo"]) == {b + 1}
         assert np.all(t["flags"] == uvt.TILE_LOWER | uvt.TILE_ACCUM)

@@ -10,6 +10,9 @@ def test_builtin_choices(uvt):
     assert uvt.tune_choice(4096, 16) == "128x32"
     assert uvt.tune_choice(300, 500) == "64x64"
     assert uvt.tune_choice(4096, 64) == "64x64"
+    # alone, fewer than 512 tiles of 128x128 would leave CUs half empty
+    assert uvt.tune_choice(2048, 2048) == "64x64"
+    assert uvt.tune_choice(16384, 16384) == "128x128"
 
 
 def test_table_lookup(uvt, tmp_path):

@@ -45,6 +45,10 @@ int32_t uvt_default_cfg(int64_t n, int64_t m)
 {
     if (m <= 48) return UVT_CFG_128x32;
     if (n <= 512 || m <= 96) return UVT_CFG_64x64;
+    /* A problem alone on the GPU: 128x128 tiles run two workgroups per CU, so
+     * with fewer than 2 x 256 of them (MI355X) half-empty CUs cost more than
+     * the smaller tile's lower efficiency. */
+    if (((n + 127) / 128) * ((m + 127) / 128) < 2 * 256) return UVT_CFG_64x64;
     return UVT_CFG_128x128;
 }
 
