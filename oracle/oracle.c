@@ -257,6 +257,41 @@ void orc_trmm_rows_dd(int64_t n, int64_t m, const double *A, int64_t lda, const 
     }
 }
 
+/* Rows of C = tril(A) * B in double-double for the seeded inputs of an n x n
+ * A and n x m B, generating every element on the fly (no n x n host copy):
+ * the same sums as orc_trmm_rows_dd over orc_fill_seeded_A/B. */
+void orc_seeded_rows_dd(int64_t n, int64_t m, uint64_t seed, int spread, const int64_t *rows,
+                        int64_t nrows, double *out, int64_t ldo)
+{
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t r = 0; r < nrows; r++) {
+        const int64_t i = rows[r];
+        double *hi = (double *)calloc((size_t)m, sizeof(double));
+        double *lo = (double *)calloc((size_t)m, sizeof(double));
+        if (!hi || !lo) {
+            free(hi);
+            free(lo);
+            for (int64_t j = 0; j < m; j++) out[r * ldo + j] = NAN;
+            continue;
+        }
+        for (int64_t k = 0; k <= i; k++) {
+            const double a = orc_seeded_value(seed, 1u, (uint64_t)(i * n + k), spread, i == k);
+            for (int64_t j = 0; j < m; j++) {
+                const double b = orc_seeded_value(seed, 2u, (uint64_t)(k * m + j), spread, 0);
+                double p = a * b;
+                double pe = fma(a, b, -p);
+                double sum, e;
+                two_sum(hi[j], p, &sum, &e);
+                hi[j] = sum;
+                lo[j] += e + pe;
+            }
+        }
+        for (int64_t j = 0; j < m; j++) out[r * ldo + j] = hi[j] + lo[j];
+        free(hi);
+        free(lo);
+    }
+}
+
 /* ---- host BLAS (MKL) loaded at run time, for the stage-loop replay ---- */
 typedef void (*dgemm_fn)(int, int, int, int, int, int, double, const double *, int, const double *,
                          int, double, double *, int);

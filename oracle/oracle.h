@@ -68,6 +68,11 @@ void orc_trmm_rows_dd(int64_t n, int64_t m, const double *A, int64_t lda, const 
  * libmkl_rt, loaded at run time); blas==0 or no MKL uses the C loops. */
 int orc_stage_loop(int64_t n, int64_t m, uint32_t p, int balanced, const double *A, int64_t lda,
                    const double *B, int64_t ldb, double *C, int64_t ldc, int blas);
+/* Rows of C for the seeded inputs (n x n A, n x m B), elements generated on
+ * the fly, summed in double-double: orc_trmm_rows_dd without host copies of
+ * A and B (for orders whose A does not fit in host memory). */
+void orc_seeded_rows_dd(int64_t n, int64_t m, uint64_t seed, int spread, const int64_t *rows,
+                        int64_t nrows, double *out, int64_t ldo);
 /* 1 if MKL's CBLAS could be loaded; fills a version string. */
 int orc_blas_available(char *version, int cap);
 /* Makes the CBLAS library at `path` the one orc_stage_loop calls (a

@@ -43,6 +43,7 @@ class Oracle:
         L.orc_stage_loop.argtypes = [i64, i64, u32, i32, P, i64, P, i64, P, i64, i32]
         L.orc_blas_available.argtypes = [c.c_char_p, i32]
         L.orc_blas_use.argtypes = [c.c_char_p]
+        L.orc_seeded_rows_dd.argtypes = [i64, i64, u64, i32, c.POINTER(i64), i64, P, i64]
         L.orc_ranks_stage_loop.argtypes = [i64, i64, u32, i32, i32, P, i64, P, i64, P, i64, i32]
         L.orc_ranks_stage_loop.restype = c.c_double
         L.orc_inf_norm.argtypes = [i64, i64, P, i64, P]
@@ -103,6 +104,14 @@ class Oracle:
         self.L.orc_trmm_rows_dd(n, m, self._p(A), A.shape[1], self._p(B), m,
                                 rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(rows),
                                 self._p(out), m)
+        return out
+
+    def seeded_rows_dd(self, n, m, seed, spread, rows):
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        out = np.empty((len(rows), m))
+        self.L.orc_seeded_rows_dd(n, m, seed, spread,
+                                  rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(rows),
+                                  self._p(out), m)
         return out
 
     def stage_loop(self, A, B, p, mapping, blas=0):

@@ -109,3 +109,26 @@ def test_twelve_moduli_large_k(gpu, uvt, oracle):
     say("rel_fro 12 moduli", err)
     assert err <= TOL
 
+
+
+def test_max_order_65536_split(gpu, oracle):
+    """Split path at N = 65536 (reduction depth 65280, the int32 accumulation
+    bound's range) with 13 moduli: sampled rows against double-double sums."""
+    n, m = 65536, 64
+    say("split max order", n, m)
+    ws = gpu.split_workspace(n, m, 13)
+    dA, dB, dC, dW = gpu.alloc(n * n * 8), gpu.alloc(n * m * 8), gpu.alloc(n * m * 8), gpu.alloc(ws)
+    try:
+        gpu.fill_seeded(n, m, 92, 1, dA, n, dB, m)
+        gpu.trmm_split(n, m, 13, dA, n, dB, m, dC, m, dW)
+        gpu.sync()
+        C = dC.download((n, m))
+    finally:
+        for b in (dA, dB, dC, dW):
+            b.free()
+    rows = [0, 255, 256, 257, 30000, 65279, 65280, 65535]
+    ref = oracle.seeded_rows_dd(n, m, 92, 1, rows)
+    got = C[rows]
+    err = rel_fro(got, ref)
+    say("rel_fro split 65536", err)
+    assert err <= TOL

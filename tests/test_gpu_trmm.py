@@ -272,3 +272,27 @@ def test_cli_split_precision_norm(uvt):
                              text=True, timeout=120, cwd=cwd)
         assert out.returncode == 0, out.stderr
         assert f"\nNorm: {want}\n" in out.stdout, out.stdout
+
+
+def test_max_order_65536(gpu, uvt, oracle):
+    """N = 65536 (A alone is 34 GB; row offsets pass 2^32 elements), balanced
+    over 16 bands, trapezoid, M = 64: sampled rows against double-double sums
+    of the seeded inputs generated on the host element by element."""
+    n, m = 65536, 64
+    say("max order", n, m)
+    dA, dB, dC = gpu.alloc(n * n * 8), gpu.alloc(n * m * 8), gpu.alloc(n * m * 8)
+    try:
+        gpu.fill_seeded(n, m, 91, 1, dA, n, dB, m)
+        plan = uvt.Plan(n, m, 16, "balanced", "trapezoid")
+        gpu.trmm_f64(plan, dA, n, dB, m, dC, m)
+        gpu.sync()
+        bands = plan.bands()
+        plan.close()
+        C = dC.download((n, m))
+    finally:
+        for b in (dA, dB, dC):
+            b.free()
+    rows = sorted({0, 1, 32767, 32768, 65535} | {b for b, _ in bands[1:4]} | {b - 1 for b, _ in bands[1:4]})
+    ref = oracle.seeded_rows_dd(n, m, 91, 1, rows)
+    for r, i in enumerate(rows):
+        assert rel_fro(C[i], ref[r]) <= TOL, i

@@ -215,3 +215,17 @@ def test_stage_loop_blas_path(oracle):
     rc1, C1 = oracle.stage_loop(A, B, 3, "balanced", blas=1)
     assert rc0 == rc1 == 0
     assert rel_fro(C1, C0) < 1e-14
+
+
+def test_seeded_rows_dd_equals_rows_dd_over_filled_inputs(oracle):
+    """The on-the-fly row oracle (used at orders whose A does not fit in host
+    memory) equals orc_trmm_rows_dd over orc_fill_seeded_A/B bit for bit:
+    same elements, same double-double sums in the same order."""
+    n, m = 700, 37
+    rows = [0, 1, 255, 256, 699]
+    for spread in (0, 1):
+        A = oracle.seeded_A(n, 9, spread)
+        B = oracle.seeded_B(n, m, 9, spread)
+        want = oracle.trmm_rows_dd(A, B, rows)
+        got = oracle.seeded_rows_dd(n, m, 9, spread, rows)
+        np.testing.assert_array_equal(got, want)
