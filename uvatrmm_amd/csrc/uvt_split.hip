@@ -46,10 +46,11 @@ namespace {
 constexpr int kMaxMods = 16;
 constexpr int kMaxSlices = 14;  // the two 64-bit halves of the reconstruction stay below 2^56
 constexpr int kTileI = 256;  // int8 tile rows = row-tile granularity of the box/diagonal split
-constexpr int kI8BK = 128;   // k bytes per LDS stage
+constexpr int kI8BK = 64;    // k bytes per LDS stage (one v_mfma_i32_16x16x64_i8 step)
 constexpr int kI8S = kI8BK + 16;  // LDS row stride (bytes): conflict-free 16-byte fragment reads
 constexpr int kI8Stage = kTileI * kI8S;  // one B stage
-constexpr int kFrag = 1024;       // one MFMA operand fragment: 32 rows x 32 k bytes
+constexpr int kFrag = 1024;       // one MFMA operand fragment: 16 rows x 64 k bytes
+constexpr int kFragSkew = 256;    // extra bytes per 16-row block of A' (see a_block_stride)
 
 // Pairwise coprime moduli, largest first (distinct prime factors).
 constexpr int kModuli[kMaxMods] = {256, 255, 253, 251, 247, 241, 239, 233,
@@ -90,7 +91,6 @@ struct Desc {
 };
 
 typedef int v4i __attribute__((ext_vector_type(4)));
-typedef int v16i __attribute__((ext_vector_type(16)));
 
 // 16 bytes at base + off as a global load (scalar base + 32-bit lane offset;
 // base comes from a descriptor, see gbl()).
@@ -119,6 +119,14 @@ __device__ __forceinline__ int64_t row_box_end(int64_t i) { return (i / kTileI) 
 // Residue arrays are padded with zero rows to whole 256-row tiles, so every
 // product tile loads without bounds checks.
 __host__ __device__ __forceinline__ int64_t pad_tile(int64_t x) { return (x + kTileI - 1) / kTileI * kTileI; }
+// Bytes from one 16-row block of A' fragments to the next: kmax/64 fragments
+// plus a 256-byte skew.  Without it every row block starts on the same
+// 4 KiB boundary, and the waves of an XCD, which read the same k block of
+// different row blocks at once, crowd into a quarter of the L2 channels
+// (measured 3-11 % slower products).
+__host__ __device__ __forceinline__ int64_t a_block_stride(int64_t kmax) { return kmax / 64 * kFrag + kFragSkew; }
+// Bytes of A' residues per modulus.
+__host__ __device__ __forceinline__ int64_t a_mod_bytes(int64_t n, int64_t kmax) { return pad_tile(n) / 16 * a_block_stride(kmax); }
 
 // ---- scales ---------------------------------------------------------------
 
@@ -186,14 +194,14 @@ __device__ __forceinline__ int sym_residue(double a, int m, double invm)
 }
 
 // Rows of A'.  The residues go out in MFMA operand order: for row block
-// rb = i/32 and k block kb = k/32, fragment (rb, kb) is 1024 contiguous bytes
-// where lane l = i%32 + 32*((k%32)/16) holds k bytes 16*(l/32) .. +16 of row
-// rb*32 + l%32, i.e. the 16 bytes lane l feeds v_mfma_i32_32x32x32_i8.  A
-// wave of the product kernel then loads a fragment with one coalesced
-// 1 KiB read.  A workgroup covers 32 rows (rows >= 256) x 128 k, i.e. four
-// fragments; thread t takes 16 consecutive k of row t/8, so a wave reads 8
-// rows x 1 KiB of A and writes 8 whole 128-byte lines per modulus.  Rows
-// n .. pad_tile(n) are zeros.
+// rb = i/16 and k block kb = k/64, fragment (rb, kb) is 1024 contiguous bytes
+// at rb * a_block_stride + kb * 1024, where lane l = i%16 + 16*((k%64)/16)
+// holds k bytes 16*(l/16) .. +16 of row rb*16 + l%16, i.e. the 16 bytes lane
+// l feeds v_mfma_i32_16x16x64_i8.  A wave of the product kernel then loads a
+// fragment with one coalesced 1 KiB read.  A workgroup covers 32 rows
+// (rows >= 256) x 128 k, i.e. four fragments; thread t takes 16 consecutive
+// k of row t/8, so a wave reads 8 rows x 1 KiB of A and writes 8 whole
+// 128-byte lines per modulus.  Rows n .. pad_tile(n) are zeros.
 __global__ __launch_bounds__(256) void k_resid_a(Desc one, const Desc* table, const int* lut,
                                                  int nprob, Mods mods)
 {
@@ -210,8 +218,9 @@ __global__ __launch_bounds__(256) void k_resid_a(Desc one, const Desc* table, co
     const UVT_GLOBAL double* a = gbl(d.A) + i * d.lda + k0;
 #pragma unroll
     for (int e = 0; e < 16; ++e) v[e] = iv ? rint(ldexp(a[e], al)) : 0.0;
-    const int64_t kbn = d.kmax / 32, mod_stride = pad_tile(d.n) * d.kmax;
-    UVT_GLOBAL int8_t* out = gbl(d.Ar) + ((rb * kbn + kq * 4 + (c >> 1)) * 64 + (t >> 3) + 32 * (c & 1)) * 16;
+    const int64_t mod_stride = a_mod_bytes(d.n, d.kmax);
+    UVT_GLOBAL int8_t* out = gbl(d.Ar) + (i >> 4) * a_block_stride(d.kmax) + (kq * 2 + (c >> 2)) * kFrag +
+                             ((i & 15) + 16 * (c & 3)) * 16;
     for (int l = 0; l < mods.n; ++l) {
         uint32_t q4[4];
 #pragma unroll
@@ -265,78 +274,91 @@ __global__ __launch_bounds__(256) void k_resid_b(Desc one, const Desc* table, co
 
 // ---- int8 modular products ---------------------------------------------------
 
-// The k loop of one 256 x 256 tile; 8 waves of 64 x 128, each a 2 x 4 block
-// of v_mfma_i32_32x32x32_i8.  A' fragments come straight from memory into
-// registers (fragment order, one coalesced 1 KiB load each), one k stage
-// (128 bytes) ahead; B' panels go through a double-buffered LDS tile.  The
-// loop is unrolled by two so both register buffers have fixed names.  The
-// residue arrays are padded to whole tiles, so no load needs a bounds check.
-__device__ __forceinline__ void i8_tile_loop(const int8_t* __restrict__ Af, int64_t kbn,
+// The k loop of one 256 x 256 tile; 8 waves of 64 x 128, each a 4 x 8 block
+// of v_mfma_i32_16x16x64_i8.  A' fragments come straight from memory into
+// registers (fragment order, one coalesced 1 KiB load each); B' panels go
+// through a double-buffered LDS tile of 64 k.  Loads run two k stages ahead:
+// the products are bound by memory latency, not bandwidth (one stage of
+// prefetch left every CU waiting on about 48 KiB in flight, measured), so
+// A' has two register sets, even and odd stages, and each fragment slot is
+// refilled for stage kt + 2 as soon as its products are issued; B' has two
+// register sets and is written to LDS the stage before its use.  The loop
+// is unrolled by two so every buffer has a fixed name.  The residue arrays
+// are padded to whole tiles, so no load needs a bounds check.
+__device__ __forceinline__ void i8_tile_loop(const int8_t* __restrict__ Af, int64_t astr,
                                              const int8_t* __restrict__ B, int64_t ldb_r, int K,
-                                             int8_t* smem, v16i (&acc)[2][4])
+                                             int8_t* smem, v4i (&acc)[4][8])
 {
-    constexpr int KS = kI8BK / 32;
     // Addresses are a wave-uniform base (scalar registers) plus one 32-bit
     // per-lane offset, which keeps the vector registers for the operands.
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave >> 1, wn = wave & 1;
-    // this lane's 16 bytes of fragment (row block 2*wm + i, k block kb)
-    const int8_t* Awave = Af + (int64_t)(2 * wm) * kbn * kFrag;
+    // this lane's 16 bytes of fragment (row block 4*wm + i, k block kt)
+    const int8_t* Awave = Af + (int64_t)(4 * wm) * astr;
     const uint32_t aoff = (uint32_t)lane * 16;
-    // B panel rows tid/8 + 64 q, 16 bytes at (tid%8)*16 of the stage
-    const uint32_t boff = (uint32_t)(tid >> 3) * (uint32_t)ldb_r + (uint32_t)(tid & 7) * 16;
-    const uint32_t soff = (uint32_t)(tid >> 3) * kI8S + (uint32_t)(tid & 7) * 16;
-    v4i a0[KS][2], a1[KS][2], rb[4];
-    auto load_a = [&](int k0, v4i (&d)[KS][2]) {
+    // B panel rows tid/4 + 128 q, 16 bytes at (tid%4)*16 of the stage
+    const uint32_t boff = (uint32_t)(tid >> 2) * (uint32_t)ldb_r + (uint32_t)(tid & 3) * 16;
+    const uint32_t soff = (uint32_t)(tid >> 2) * kI8S + (uint32_t)(tid & 3) * 16;
+    v4i aE[4], aO[4], bE[2], bO[2];
+    auto load_a = [&](int kt, int i) { return ld_global16(Awave + (int64_t)i * astr + (int64_t)kt * kFrag, aoff); };
+    auto load_b = [&](int kt, v4i (&d)[2]) {
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks)
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-                d[ks][i] = ld_global16(Awave + ((int64_t)i * kbn + (k0 >> 5) + ks) * kFrag, aoff);
+        for (int q = 0; q < 2; ++q) d[q] = ld_global16(B + (int64_t)q * 128 * ldb_r + kt * kI8BK, boff);
     };
-    auto load_b = [&](int k0) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-            rb[q] = ld_global16(B + (int64_t)q * 64 * ldb_r + k0, boff);
-    };
-    auto store_b = [&](int buf) {
+    auto store_b = [&](int buf, const v4i (&d)[2]) {
         int8_t* bs = smem + buf * kI8Stage + soff;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) *reinterpret_cast<v4i*>(bs + q * 64 * kI8S) = rb[q];
+        for (int q = 0; q < 2; ++q) *reinterpret_cast<v4i*>(bs + q * 128 * kI8S) = d[q];
     };
-    auto compute = [&](int buf, v4i (&a)[KS][2]) {
-        const int8_t* bs = smem + buf * kI8Stage + (wn * 128 + (lane & 31)) * kI8S + (lane >> 5) * 16;
+    // Products of stage kt from LDS buffer buf; refills a for stage kn.
+    auto compute = [&](int buf, v4i (&a)[4], int kn) {
+        const int8_t* bs = smem + buf * kI8Stage + (wn * 128 + (lane & 15)) * kI8S + (lane >> 4) * 16;
+        v4i b[8];
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-            v4i b[4];
+        for (int j = 0; j < 8; ++j) b[j] = *reinterpret_cast<const v4i*>(bs + j * 16 * kI8S);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const v4i*>(bs + j * 32 * kI8S + ks * 32);
+        for (int i = 0; i < 4; ++i) {
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[ks][i], b[j], acc[i][j], 0, 0, 0);
+            for (int j = 0; j < 8; ++j) acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], acc[i][j], 0, 0, 0);
+            a[i] = load_a(kn, i);
         }
     };
-    const int nk = K / kI8BK;  // K is a multiple of 256, so nk is even
-    load_a(0, a0);
-    load_b(0);
-    store_b(0);
+    // Every load and store is unconditional (past the end they fetch the
+    // last stage again and the values go unused): a load under a branch
+    // makes the compiler's wait counts at the join assume the branch not
+    // taken, i.e. wait for the loads just issued.
+    // The prologue issues its loads in the loop's order (pinned): the wait
+    // counts at the loop head must hold for the first iteration too.
+    const int nk = K / kI8BK;  // K is a multiple of 256, so nk is a multiple of 4
+    load_b(0, bE);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) aE[i] = load_a(0, i);
+    __builtin_amdgcn_sched_barrier(0);
+    load_b(1, bO);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) aO[i] = load_a(1, i);
+    __builtin_amdgcn_sched_barrier(0);
+    store_b(0, bE);
     __syncthreads();
+    // The scheduling barriers keep the B' loads at the head of their stage
+    // and each LDS store after the products of its stage, so the B' data of
+    // a stage has two stages to arrive.
     for (int kt = 0; kt < nk; kt += 2) {
-        load_a((kt + 1) * kI8BK, a1);
-        load_b((kt + 1) * kI8BK);
-        compute(0, a0);
-        store_b(1);
+        const int k2 = min(kt + 2, nk - 1), k3 = min(kt + 3, nk - 1);
+        load_b(k2, bE);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(0, aE, k2);
+        __builtin_amdgcn_sched_barrier(0);
+        store_b(1, bO);
         __syncthreads();
-        const bool more = kt + 2 < nk;
-        if (more) {
-            load_a((kt + 2) * kI8BK, a0);
-            load_b((kt + 2) * kI8BK);
-        }
-        compute(1, a1);
-        if (more) store_b(0);
+        load_b(k3, bO);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(1, aO, k3);
+        __builtin_amdgcn_sched_barrier(0);
+        store_b(0, bE);
         __syncthreads();
     }
 }
@@ -369,34 +391,34 @@ __global__ __launch_bounds__(512, 1) void k_gemm_i8(Desc one, const Desc* table,
     if (rt < 1 || ct >= nct) return;  // padding of the last groups (uniform per block)
     const int64_t r0 = (int64_t)rt * kTileI, c0 = (int64_t)ct * kTileI;
     const int64_t n = UVT_FIELD(n), m = UVT_FIELD(m), kmax = UVT_FIELD(kmax);
-    const int64_t kbn = kmax / 32;
-    const int8_t* Af = UVT_FIELD(Ar) + l * (pad_tile(n) * kmax) + (r0 / 32) * kbn * kFrag;
+    const int64_t astr = a_block_stride(kmax);
+    const int8_t* Af = UVT_FIELD(Ar) + l * a_mod_bytes(n, kmax) + (r0 / 16) * astr;
     const int8_t* B = UVT_FIELD(Br) + l * (pad_tile(m) * kmax) + c0 * kmax;
     UVT_GLOBAL uint8_t* const Rout = gbl(UVT_FIELD(R));
 #undef UVT_FIELD
 
-    v16i acc[2][4];
+    v4i acc[4][8];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = v16i{0};
-    i8_tile_loop(Af, kbn, B, kmax, (int)r0, smem, acc);
+        for (int j = 0; j < 8; ++j) acc[i][j] = v4i{0, 0, 0, 0};
+    i8_tile_loop(Af, astr, B, kmax, (int)r0, smem, acc);
 
-    // D layout of v_mfma_i32_32x32x32_i8 (measured on gfx950): lane l, element
-    // v holds row 8*(v/4) + 4*(l/32) + v%4, column l%32.
+    // D layout of v_mfma_i32_16x16x64_i8: lane l, element v holds row
+    // 4*(l/16) + v, column l%16.
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int mod = mods.m[l];
     const float invf = mods.invf[l];
     UVT_GLOBAL uint8_t* out = Rout + l * (n * m);
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int64_t col = c0 + wn * 128 + j * 32 + (lane & 31);
+        for (int j = 0; j < 8; ++j) {
+            const int64_t col = c0 + wn * 128 + j * 16 + (lane & 15);
 #pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const int64_t row = r0 + wm * 64 + i * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+            for (int v = 0; v < 4; ++v) {
+                const int64_t row = r0 + wm * 64 + i * 16 + 4 * (lane >> 4) + v;
                 if (row < n && col < m) {
                     const int x = acc[i][j][v];
                     int r = x - mod * (int)floorf((float)x * invf);
@@ -644,7 +666,7 @@ size_t problem_bytes(int64_t n, int64_t m, int nmod)
     o = align256(o + (size_t)n * 4);
     o = align256(o + (size_t)m * 4);
     o = align256(o + (size_t)chunks * m * 8);
-    o = align256(o + (size_t)nmod * pad_tile(n) * kmax);
+    o = align256(o + (size_t)nmod * a_mod_bytes(n, kmax));
     o = align256(o + (size_t)nmod * pad_tile(m) * kmax);
     o = align256(o + (size_t)nmod * n * m);
     return o;
@@ -698,7 +720,7 @@ void describe(const uvt_split_problem* probs, int count, int nmod, char* work, D
             d.part = (double*)(w + o);
             o = align256(o + (size_t)d.chunks * q.m * 8);
             d.Ar = (int8_t*)(w + o);
-            o = align256(o + (size_t)nmod * pad_tile(q.n) * d.kmax);
+            o = align256(o + (size_t)nmod * a_mod_bytes(q.n, d.kmax));
             d.Br = (int8_t*)(w + o);
             o = align256(o + (size_t)nmod * pad_tile(q.m) * d.kmax);
             d.R = (uint8_t*)(w + o);
