@@ -91,6 +91,7 @@ struct Desc {
 };
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
 
 // 16 bytes at base + off as a global load (scalar base + 32-bit lane offset;
 // base comes from a descriptor, see gbl()).
@@ -199,25 +200,45 @@ __device__ __forceinline__ int sym_residue(double a, int m, double invm)
 // holds k bytes 16*(l/16) .. +16 of row rb*16 + l%16, i.e. the 16 bytes lane
 // l feeds v_mfma_i32_16x16x64_i8.  A wave of the product kernel then loads a
 // fragment with one coalesced 1 KiB read.  A workgroup covers 32 rows
-// (rows >= 256) x 128 k, i.e. four fragments; thread t takes 16 consecutive
-// k of row t/8, so a wave reads 8 rows x 1 KiB of A and writes 8 whole
-// 128-byte lines per modulus.  Rows n .. pad_tile(n) are zeros.
+// (rows >= 256) x 128 k, i.e. four fragments.  The scaled, rounded rows go
+// through LDS: read from A with the 8 lanes of a row on 128 contiguous bytes
+// (16 per lane), then taken back as 16 consecutive k per thread (row t/8),
+// so a wave writes 8 whole 128-byte lines per modulus.  Rows
+// n .. pad_tile(n) are zeros.
 __global__ __launch_bounds__(256) void k_resid_a(Desc one, const Desc* table, const int* lut,
                                                  int nprob, Mods mods)
 {
+    __shared__ __attribute__((aligned(16))) double tv[32][128 + 4];
     const Desc d = fetch(one, table, lut, nprob);
     const int64_t wb = (int64_t)blockIdx.x - d.start[K_RA];
     const int64_t rb = kTileI / 32 + wb / d.gxa, kq = wb % d.gxa;
     if (kq * 128 >= row_box_end(rb * 32)) return;  // boxes end on multiples of 256
-    const int t = threadIdx.x, c = t & 7;
-    const int64_t i = rb * 32 + (t >> 3);
-    const int64_t k0 = kq * 128 + c * 16;
+    const int t = threadIdx.x, c = t & 7, r = t >> 3;
+    const int64_t i = rb * 32 + r;
     const bool iv = i < d.n;
     const int al = iv ? gbl(d.alpha)[i] : 0;
-    double v[16];
-    const UVT_GLOBAL double* a = gbl(d.A) + i * d.lda + k0;
+    {
+        const UVT_GLOBAL double* a = gbl(d.A) + i * d.lda + kq * 128 + 2 * c;
 #pragma unroll
-    for (int e = 0; e < 16; ++e) v[e] = iv ? rint(ldexp(a[e], al)) : 0.0;
+        for (int e = 0; e < 8; ++e) {
+            d2 x = {0.0, 0.0};
+            if (iv) {
+                if (d.vec)  // 16-byte aligned: A and lda even
+                    x = *reinterpret_cast<const UVT_GLOBAL d2*>(a + 16 * e);
+                else
+                    x = d2{a[16 * e], a[16 * e + 1]};
+            }
+            *reinterpret_cast<d2*>(&tv[r][16 * e + 2 * c]) = d2{rint(ldexp(x.x, al)), rint(ldexp(x.y, al))};
+        }
+    }
+    __syncthreads();
+    double v[16];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const d2 x = *reinterpret_cast<const d2*>(&tv[r][16 * c + 2 * e]);
+        v[2 * e] = x.x;
+        v[2 * e + 1] = x.y;
+    }
     const int64_t mod_stride = a_mod_bytes(d.n, d.kmax);
     UVT_GLOBAL int8_t* out = gbl(d.Ar) + (i >> 4) * a_block_stride(d.kmax) + (kq * 2 + (c >> 2)) * kFrag +
                              ((i & 15) + 16 * (c & 3)) * 16;
