@@ -59,15 +59,15 @@ def test_cli_mapping_abort_codes(uvt):
 def test_serve_request_bytes(uvt):
     """Arena bytes of a request: A, B and C (256-byte aligned pieces), plus the
     split workspace when a split-class request takes the int8 path (where it
-    is faster on its own: m >= 256 and n >= 6144, or 2048 <= n with
-    m <= 1024); otherwise it runs in FP64 and needs none."""
+    is faster on its own: m >= 1024 and n >= 4096, or m >= 512 and
+    n >= 6144); otherwise it runs in FP64 and needs none."""
     a = lambda x: (x + 255) // 256 * 256
     for n, m in [(100, 7), (768, 63), (2000, 500)]:
         assert uvt.serve_request_bytes(n, m, uvt.PREC_F64) == a(n * n * 8) + 2 * a(n * m * 8)
-    for n, m in [(2000, 500), (8192, 128), (4096, 2048)]:  # FP64 is faster
+    for n, m in [(2000, 500), (8192, 128), (3072, 4096), (4096, 512), (6144, 256)]:  # FP64 is faster
         assert uvt.serve_request_bytes(n, m, uvt.PREC_SPLIT) == \
             uvt.serve_request_bytes(n, m, uvt.PREC_F64)
-    for n, m in [(2048, 256), (4096, 1024), (8192, 4096)]:  # int8 is faster
+    for n, m in [(4096, 1024), (6144, 512), (8192, 4096)]:  # int8 is faster
         ws = ctypes.c_size_t()
         assert uvt.lib().uvt_trmm_split_workspace(n, m, 13, ctypes.byref(ws)) == uvt.OK
         assert uvt.serve_request_bytes(n, m, uvt.PREC_SPLIT) == \

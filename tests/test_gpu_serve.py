@@ -53,7 +53,8 @@ def serve(uvt, gpu, reqs, budget, slots):
 
 
 def test_serve_alone_equals_mix(uvt, gpu, oracle):
-    reqs = mixed_requests(40, seed=11)
+    # the last two are split-class requests large enough for the int8 path
+    reqs = mixed_requests(40, seed=11) + [(4096, 1024, 1, 6001), (6144, 640, 1, 6002)]
     say("requests", len(reqs), "alone")
     alone, st1 = serve(uvt, gpu, reqs, 8 * GiB, 1)
     assert st1["peak_inflight"] == 1
@@ -122,7 +123,7 @@ def test_serve_host_arrays(uvt, gpu, oracle):
 def test_serve_device_operands(uvt, gpu, oracle):
     """Operands already resident in HBM are read and written in place and the
     arena is charged only for the split workspace."""
-    cases = [(1100, 96, 1), (500, 40, 0), (2048, 256, 0), (2048, 300, 1)]  # last: int8 path
+    cases = [(1100, 96, 1), (500, 40, 0), (2048, 256, 0), (4096, 1024, 1)]  # last: int8 path
     srv = uvt.Server(gpu, 1 << 30, max_slots=4)
     bufs = []
     try:

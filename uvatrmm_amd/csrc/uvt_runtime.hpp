@@ -55,13 +55,14 @@ int uvt_tune_pick(const std::vector<uvt_tune_entry>& table, int64_t n, int64_t m
 
 // Where a split-class problem (error <= 1e-12 acceptable) runs on the int8
 // path rather than in FP64, which is always at least as accurate.  Measured on
-// MI355X with 12 moduli (split time / FP64 time):
-//   alone (serving): the int8 path is faster from m >= 256 and n >= 2048 while
-//     m <= 1024, and for every m >= 256 from n >= 6144 (1.2x - 1.7x there);
-//     narrow B (m < 256) never pays for the residue and reconstruction passes;
+// MI355X with 12 moduli (FP64 time / split time):
+//   alone (serving): the int8 path is faster from m >= 1024 once n >= 4096
+//     (1.14x - 1.98x), and from m >= 256 at n >= 6144 only by 0-6 %, so the
+//     line is drawn at m >= 512 there; narrow B (m < 256) and n < 4096 never
+//     pay for the residue and reconstruction passes;
 //   in a batch (throughput, persistent FP64 queues): FP64 wins below n = 8192.
 inline bool uvt_split_faster_alone(int64_t n, int64_t m)
 {
-    return m >= 256 && (n >= 6144 || (n >= 2048 && m <= 1024));
+    return (n >= 4096 && m >= 1024) || (n >= 6144 && m >= 512);
 }
 inline bool uvt_split_faster_in_batch(int64_t n, int64_t m) { return n >= 8192 && m >= 256; }
