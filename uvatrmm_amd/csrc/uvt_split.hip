@@ -14,8 +14,9 @@
 //     from the 2-norms so that |A'B'| <= ||A'_i|| ||B'_j|| < M/2, where
 //     M = m_1 * ... * m_N for N pairwise coprime moduli m_l <= 256.
 //   * For each modulus, the symmetric residues of A' and B' fit in int8; the
-//     product of residues with int32 accumulation is exact (|sum| < 2^29 for
-//     K < 32768 x 4), and is reduced to a residue of A'B' mod m_l.
+//     product of residues with int32 accumulation is exact (|sum| <= 128^2 K
+//     < 2^31 for the depths K <= 130816 that n <= 131072 allows), and is
+//     reduced to a residue of A'B' mod m_l.
 //   * The N residues give the mixed-radix digits of the unique C' in
 //     (-M/2, M/2) (Garner); C' is assembled exactly in two 64-bit words,
 //     rounded once to FP64, scaled by 2^-(alpha_i+beta_j) and added to the
