@@ -17,10 +17,10 @@
 //     product of residues with int32 accumulation is exact (|sum| <= 128^2 K
 //     < 2^31 for the depths K <= 130816 that n <= 131072 allows), and is
 //     reduced to a residue of A'B' mod m_l.
-//   * The N residues give the mixed-radix digits of the unique C' in
-//     (-M/2, M/2) (Garner); C' is assembled exactly in two 64-bit words,
-//     rounded once to FP64, scaled by 2^-(alpha_i+beta_j) and added to the
-//     FP64 diagonal-block part.
+//   * The N residues determine the unique C' in (-M/2, M/2) (Chinese
+//     remainder theorem); C' is formed in FP64 from exact 40-bit limb sums,
+//     scaled by 2^-(alpha_i+beta_j) and added to the FP64 diagonal-block
+//     part.
 // The error comes only from rounding A and B to integers: about
 // sqrt(K)/||A'_i|| relative, i.e. ~1e-13 for 13 moduli at K = 32768.
 //
@@ -45,7 +45,7 @@ using namespace uvt;
 namespace {
 
 constexpr int kMaxMods = 16;
-constexpr int kMaxSlices = 14;  // the two 64-bit halves of the reconstruction stay below 2^56
+constexpr int kMaxSlices = 14;  // limb sums of the reconstruction stay below 14 * 256 * 2^40 < 2^52
 constexpr int kTileI = 256;  // int8 tile rows = row-tile granularity of the box/diagonal split
 constexpr int kI8BK = 64;    // k bytes per LDS stage (one v_mfma_i32_16x16x64_i8 step)
 constexpr int kI8S = kI8BK + 16;  // LDS row stride (bytes): conflict-free 16-byte fragment reads
@@ -482,86 +482,60 @@ __global__ __launch_bounds__(512, 4) void k_diag_f64(Desc one, const Desc* table
 
 // ---- reconstruction -----------------------------------------------------------
 
-// x mod m for 0 <= x < 2^22: with magic = ceil(2^39/m), floor(x/m) equals
-// (x * magic) >> 39 exactly on that range (the rounding up of magic adds
-// less than 2^-17 < 1/m to x/m).
-__device__ __forceinline__ uint32_t mod_magic(uint32_t x, uint32_t m, uint32_t magic)
-{
-    return x - (__umulhi(x, magic) >> 7) * m;
-}
-
-// Garner tables for the first N moduli, evaluated at compile time so that the
-// reconstruction kernel sees every modulus and coefficient as an immediate.
-// With W_p = m_0 ... m_{p-1}, the mixed-radix digits of x (x = sum_p v_p W_p)
-// satisfy  v_l = (r_l q_l + sum_{p<l} v_p c_{p,l}) mod m_l  where
-// q_l = W_l^-1 mod m_l and c_{p,l} = m_l - (W_p q_l mod m_l): one modular
-// reduction per digit, and every partial sum stays below 14 * 255^2 < 2^20.
+// Chinese remainder constants for the first N moduli, evaluated at compile
+// time (128-bit integers) so that the reconstruction kernel sees them as
+// immediates.  With M = m_0 ... m_{N-1}, x = sum_l r_l e_l mod M for
+// e_l = (M/m_l) * ((M/m_l)^-1 mod m_l) < M.  Every e_l and M are split in
+// three 40-bit limbs, so that sum_l r_l E_{l,k} < 14 * 256 * 2^40 < 2^52 stays
+// an exact FP64 integer for each limb k.
 template <int N>
-struct GarnerTab {
-    static constexpr int split = N - 1 < 7 ? N - 1 : 7;  // digits [0, split) form the low word
-    uint32_t q[N];
-    uint32_t c[N][N];
-    uint32_t magic[N];  // ceil(2^39 / m_l): x mod m_l for x < 2^22
-    uint64_t wlo, qhi;  // m_0 ... m_{split-1};  m_split ... m_{N-1}
-    double wlo_hi, wlo_lo;
-    constexpr GarnerTab() : q(), c(), magic(), wlo(1), qhi(1), wlo_hi(0.0), wlo_lo(0.0)
+struct CrtTab {
+    double e[3][N];      // limbs of e_l: e_l = e[2] 2^80 + e[1] 2^40 + e[0]
+    double mk[3];        // limbs of M
+    double inv_mhi;      // 1 / (M / 2^80)
+    constexpr CrtTab() : e(), mk(), inv_mhi(0.0)
     {
+        using u128 = unsigned __int128;
+        const u128 mask = ((u128)1 << 40) - 1;
+        u128 M = 1;
+        for (int l = 0; l < N; ++l) M *= (u128)kModuli[l];
         for (int l = 0; l < N; ++l) {
             const uint32_t ml = (uint32_t)kModuli[l];
-            uint32_t wp[N] = {};
-            uint32_t w = 1 % ml;  // W_p mod m_l
-            for (int p = 0; p < l; ++p) {
-                wp[p] = w;
-                w = (w * (uint32_t)kModuli[p]) % ml;
-            }
+            const u128 W = M / ml;
+            const uint32_t w = (uint32_t)(W % ml);
             uint32_t inv = 1;
             for (uint32_t x = 1; x < ml; ++x)
                 if ((w * x) % ml == 1) {
                     inv = x;
                     break;
                 }
-            q[l] = inv;
-            for (int p = 0; p < l; ++p) c[p][l] = ml - (wp[p] * inv) % ml;
-            magic[l] = (uint32_t)(((1ull << 39) + ml - 1) / ml);
+            const u128 el = W * inv;
+            e[0][l] = (double)(uint64_t)(el & mask);
+            e[1][l] = (double)(uint64_t)((el >> 40) & mask);
+            e[2][l] = (double)(uint64_t)(el >> 80);
         }
-        for (int l = 0; l < split; ++l) wlo *= (uint64_t)kModuli[l];
-        for (int l = split; l < N; ++l) qhi *= (uint64_t)kModuli[l];
-        wlo_hi = (double)wlo;
-        wlo_lo = (double)((int64_t)wlo - (int64_t)(uint64_t)wlo_hi);
+        mk[0] = (double)(uint64_t)(M & mask);
+        mk[1] = (double)(uint64_t)((M >> 40) & mask);
+        mk[2] = (double)(uint64_t)(M >> 80);
+        const double two40 = 1099511627776.0;
+        inv_mhi = 1.0 / (mk[2] + mk[1] / two40 + mk[0] / (two40 * two40));
     }
 };
 
-// sum_{l in [A, B)} v_l m_A ... m_{l-1} (exact; < 2^56 for B - A <= 7):
-// Horner over chunks of three digits, each chunk exact in 24 bits.
-template <int A, int B>
-__device__ __forceinline__ uint64_t digits_value(const uint32_t* v)
-{
-    constexpr int nch = (B - A + 2) / 3;
-    uint64_t w = 0;
-#pragma unroll
-    for (int ch = nch - 1; ch >= 0; --ch) {
-        const int lo = A + 3 * ch, hi = lo + 3 < B ? lo + 3 : B;
-        uint32_t cv = 0, P = 1;
-#pragma unroll
-        for (int l = hi - 1; l >= lo; --l) cv = __umul24(cv, (uint32_t)kModuli[l]) + v[l];
-#pragma unroll
-        for (int l = lo; l < hi; ++l) P *= (uint32_t)kModuli[l];
-        w = w * P + cv;
-    }
-    return w;
-}
-
 // C[i][j] += C'_ij * 2^-(alpha_i + beta_j) for rows past the first 256-row
-// tile, with N moduli.  Per element: the mixed-radix digits of C' from its
-// residues (GarnerTab), then x = X_lo + W X_hi with X_lo = digits [0, split)
-// and X_hi = the rest, both exact in 64 bits, and the signed value (x, or
-// x - M when x > M/2) rounded once to FP64 with a double-word W.  A
-// workgroup takes 1024 columns of a row, four per thread.
+// tile, with N moduli.  Per element, all in FP64 and exact up to the last
+// step: the limb sums S_k = sum_l r_l E_{l,k}; q = rint(S / M), which the
+// scale target makes unambiguous (|C'| < M/2 by a relative margin above
+// 1e-6, and S / M is formed to about 2^-40); X_k = S_k - q M_k; a carry pass
+// that brings X_0, X_1 into [0, 2^40); and C' = (X_2 2^40 + X_1) 2^40 + X_0
+// with two roundings, exact whenever C' is an FP64 number.  A workgroup
+// takes 1024 columns of a row, four per thread.
 template <int N>
 __global__ __launch_bounds__(256) void k_reconstruct(Desc one, const Desc* table, const int* lut,
                                                      int nprob)
 {
-    constexpr GarnerTab<N> G{};
+    constexpr CrtTab<N> T{};
+    constexpr double two40 = 1099511627776.0, inv40 = 1.0 / two40, inv80 = inv40 * inv40;
     const Desc d = fetch(one, table, lut, nprob);
     const int64_t wb = (int64_t)blockIdx.x - d.start[K_REC];
     const int64_t i = kTileI + wb / d.gxr;
@@ -585,34 +559,45 @@ __global__ __launch_bounds__(256) void k_reconstruct(Desc one, const Desc* table
         }
     }
     const int ai = gbl(d.alpha)[i];
+    double xs[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-        if (e >= ne) break;
-        uint32_t v[N];
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll
         for (int l = 0; l < N; ++l) {
-            uint32_t acc = __umul24(res[l][e], G.q[l]);
+            const double r = (double)res[l][e];
+            s0 = fma(r, T.e[0][l], s0);
+            s1 = fma(r, T.e[1][l], s1);
+            s2 = fma(r, T.e[2][l], s2);
+        }
+        const double q = rint(fma(s1, inv40, fma(s0, inv80, s2)) * T.inv_mhi);
+        double x0 = fma(-q, T.mk[0], s0), x1 = fma(-q, T.mk[1], s1), x2 = fma(-q, T.mk[2], s2);
+        const double c0 = floor(x0 * inv40);
+        x0 = fma(-c0, two40, x0);
+        x1 += c0;
+        const double c1 = floor(x1 * inv40);
+        x1 = fma(-c1, two40, x1);
+        x2 += c1;
+        xs[e] = fma(fma(x2, two40, x1), two40, x0);
+    }
+    // C and beta in 16-byte pieces where aligned (the row start decides, the
+    // same for the whole workgroup), else element by element.
+    UVT_GLOBAL double* crow = gbl(d.C) + i * d.ldc + j4;
+    const UVT_GLOBAL int* brow = gbl(d.beta) + j4;
+    if (ne == 4 && ((uintptr_t)crow & 15) == 0) {
+        const v4i b = *reinterpret_cast<const UVT_GLOBAL v4i*>(brow);
+        d2 c01 = *reinterpret_cast<const UVT_GLOBAL d2*>(crow);
+        d2 c23 = *reinterpret_cast<const UVT_GLOBAL d2*>(crow + 2);
+        c01.x += ldexp(xs[0], -(ai + b.x));
+        c01.y += ldexp(xs[1], -(ai + b.y));
+        c23.x += ldexp(xs[2], -(ai + b.z));
+        c23.y += ldexp(xs[3], -(ai + b.w));
+        *reinterpret_cast<UVT_GLOBAL d2*>(crow) = c01;
+        *reinterpret_cast<UVT_GLOBAL d2*>(crow + 2) = c23;
+    } else {
 #pragma unroll
-            for (int p = 0; p < l; ++p) acc += __umul24(v[p], G.c[p][l]);
-            v[l] = kModuli[l] == 256 ? (acc & 255u)
-                                     : mod_magic(acc, (uint32_t)kModuli[l], G.magic[l]);
-        }
-        const uint64_t xlo = digits_value<0, GarnerTab<N>::split>(v);
-        const uint64_t xhi = digits_value<GarnerTab<N>::split, N>(v);
-        double h, low;
-        // x > M/2 exactly:  2x - M = W (2 X_hi - Q) + 2 X_lo with 0 <= X_lo < W
-        const int64_t dq = 2 * (int64_t)xhi - (int64_t)G.qhi;
-        const bool neg = dq >= 1 || (dq == 0 && xlo > 0) || (dq == -1 && 2 * xlo > G.wlo);
-        if (neg) {  // value x - M = (X_hi - Q + 1) W + (X_lo - W)
-            h = (double)((int64_t)xhi - (int64_t)G.qhi + 1);
-            low = (double)((int64_t)xlo - (int64_t)G.wlo);
-        } else {
-            h = (double)xhi;
-            low = (double)xlo;
-        }
-        const double x = fma(h, G.wlo_hi, fma(h, G.wlo_lo, low));
-        const int64_t j = j4 + e;
-        gbl(d.C)[i * d.ldc + j] += ldexp(x, -(ai + gbl(d.beta)[j]));
+        for (int e = 0; e < 4; ++e)
+            if (e < ne) crow[e] += ldexp(xs[e], -(ai + brow[e]));
     }
 }
 
