@@ -526,10 +526,11 @@ struct CrtTab {
 // tile, with N moduli.  Per element, all in FP64 and exact up to the last
 // step: the limb sums S_k = sum_l r_l E_{l,k}; q = rint(S / M), which the
 // scale target makes unambiguous (|C'| < M/2 by a relative margin above
-// 1e-6, and S / M is formed to about 2^-40); X_k = S_k - q M_k; a carry pass
-// that brings X_0, X_1 into [0, 2^40); and C' = (X_2 2^40 + X_1) 2^40 + X_0
-// with two roundings, exact whenever C' is an FP64 number.  A workgroup
-// takes 1024 columns of a row, four per thread.
+// 1e-6, and S / M is formed to about 2^-40); X_k = S_k - q M_k; a carry that
+// brings X_0 into [0, 2^40), so that X_2 2^40 + X_1 is the integer
+// floor(C' / 2^40); and C' = (X_2 2^40 + X_1) 2^40 + X_0 with two roundings
+// (fma), exact whenever C' is an FP64 number.  A workgroup takes 1024
+// columns of a row, four per thread.
 template <int N>
 __global__ __launch_bounds__(256) void k_reconstruct(Desc one, const Desc* table, const int* lut,
                                                      int nprob)
@@ -575,9 +576,6 @@ __global__ __launch_bounds__(256) void k_reconstruct(Desc one, const Desc* table
         const double c0 = floor(x0 * inv40);
         x0 = fma(-c0, two40, x0);
         x1 += c0;
-        const double c1 = floor(x1 * inv40);
-        x1 = fma(-c1, two40, x1);
-        x2 += c1;
         xs[e] = fma(fma(x2, two40, x1), two40, x0);
     }
     // C and beta in 16-byte pieces where aligned (the row start decides, the
