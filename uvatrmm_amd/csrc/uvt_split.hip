@@ -260,11 +260,14 @@ __global__ __launch_bounds__(256) void k_resid_a(Desc one, const Desc* table, co
 }
 
 // Br[l][j][k] (B transposed) for k < kmax: 64 x 64 tiles through LDS.
-// Rows m .. pad_tile(m) are zeros.
+// Thread (j = t%64, kq = t/64) converts k = 4 (kq + 4p) + e of column j
+// (p, e < 4), so it packs whole 4-byte words of the transposed tile; the words
+// go to LDS k-word-major (consecutive lanes, consecutive banks) and come back
+// as 16-byte row pieces.  Rows m .. pad_tile(m) are zeros.
 __global__ __launch_bounds__(256) void k_resid_b(Desc one, const Desc* table, const int* lut,
                                                  int nprob, Mods mods)
 {
-    __shared__ __attribute__((aligned(16))) int8_t t[64][64 + 16];
+    __shared__ uint32_t t[16][64 + 4];  // [k word][column]
     const Desc d = fetch(one, table, lut, nprob);
     const int64_t wb = (int64_t)blockIdx.x - d.start[K_RB];
     const int64_t nkx = d.kmax / 64;
@@ -275,21 +278,28 @@ __global__ __launch_bounds__(256) void k_resid_b(Desc one, const Desc* table, co
     const int be = jv ? gbl(d.beta)[j] : 0;
     double v[16];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int64_t k = k0 + kq + 4 * q;  // < kmax: kmax is a multiple of 64
-        v[q] = jv ? rint(ldexp(gbl(d.B)[k * d.ldb + j], be)) : 0.0;
-    }
-    const int wr = threadIdx.x >> 2, ws = (threadIdx.x & 3) * 16;
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int64_t k = k0 + 4 * (kq + 4 * p) + e;  // < kmax: kmax is a multiple of 64
+            v[p * 4 + e] = jv ? rint(ldexp(gbl(d.B)[k * d.ldb + j], be)) : 0.0;
+        }
+    const int wr = threadIdx.x >> 2, ws = threadIdx.x & 3;
     const int64_t mod_stride = pad_tile(d.m) * d.kmax;
     for (int l = 0; l < mods.n; ++l) {
 #pragma unroll
-        for (int q = 0; q < 16; ++q)
-            t[jj][kq + 4 * q] = (int8_t)sym_residue(v[q], mods.m[l], mods.invd[l]);
-        __syncthreads();
-        {
-            const v4i x = *reinterpret_cast<const v4i*>(&t[wr][ws]);
-            *reinterpret_cast<UVT_GLOBAL v4i*>(gbl(d.Br) + l * mod_stride + (j0 + wr) * d.kmax + k0 + ws) = x;
+        for (int p = 0; p < 4; ++p) {
+            uint32_t x = 0;
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                x |= (uint32_t)(uint8_t)(int8_t)sym_residue(v[p * 4 + e], mods.m[l], mods.invd[l])
+                     << (8 * e);
+            t[kq + 4 * p][jj] = x;
         }
+        __syncthreads();
+        const v4i x = {(int)t[4 * ws][wr], (int)t[4 * ws + 1][wr], (int)t[4 * ws + 2][wr],
+                       (int)t[4 * ws + 3][wr]};
+        *reinterpret_cast<UVT_GLOBAL v4i*>(gbl(d.Br) + l * mod_stride + (j0 + wr) * d.kmax + k0 + 16 * ws) = x;
         __syncthreads();
     }
 }
