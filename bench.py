@@ -183,7 +183,8 @@ def batch_leg(args, h, uvt):
 def serve_leg(args, h, uvt):
     """Served stream: TRMM requests with orders log-uniform in [128, 16384], B
     columns uniform in {64..4096}, half FP64 and half split class, inputs
-    generated on the device per request, under a 200 GB HBM admission budget.
+    generated on the device per request, under a 200 GB HBM admission budget;
+    requests of m n^2 >= 5e10 (about 0.75 ms of the GPU) run one at a time.
     Capacity is measured first (a burst of requests submitted at once); the
     stream then arrives as a Poisson process at `serve_load` x capacity, and
     latency is arrival (submit) -> result ready in HBM, over the requests
@@ -200,7 +201,7 @@ def serve_leg(args, h, uvt):
 
     budget = int(args.serve_budget_gb * 1e9)
     srv = uvt.Server(h, budget, max_slots=args.serve_slots, slices=args.split_moduli or 13,
-                     table=table)
+                     table=table, big_flops=args.serve_big_flops, max_big=args.serve_max_big)
     seed = [args.seed * 1000003]
 
     def submit(req, tag):
@@ -254,6 +255,7 @@ def serve_leg(args, h, uvt):
            "arrival_rate_per_s": round(rate, 2), "load": args.serve_load,
            "capacity_per_s": round(capacity, 2), "capacity_tflops": round(burst_tflops, 4),
            "slots": args.serve_slots, "budget_gb": args.serve_budget_gb,
+           "large_request_flops": args.serve_big_flops, "max_large_in_flight": args.serve_max_big,
            "peak_arena_gb": round(st["peak_bytes"] / 1e9, 3),
            "peak_inflight": st["peak_inflight"],
            "split_share": round(float(np.mean([p for _, _, p in stream])), 3),
@@ -321,6 +323,10 @@ def main():
     ap.add_argument("--serve-capacity-reqs", type=int, default=300)
     ap.add_argument("--serve-load", type=float, default=0.7)
     ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-big-flops", type=float, default=5e10,
+                    help="m n^2 from which a served request counts as large")
+    ap.add_argument("--serve-max-big", type=int, default=1,
+                    help="large requests in flight at once (0 = no cap)")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

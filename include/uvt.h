@@ -218,12 +218,20 @@ uvt_status uvt_tune_choice(const char *tune_table, int64_t n, int64_t m, int32_t
  * arrive while others are on the GPU join them (continuous batching).  A
  * request's result does not depend on what else is in flight.  The server
  * uses only the handle's device (not its workspace), so the handle stays
- * usable from the caller's thread. */
+ * usable from the caller's thread.
+ * Large requests can be capped: with max_big > 0, at most max_big requests
+ * of m n^2 >= big_flops run at once, and smaller requests queued behind a
+ * held-back large one are admitted past it (large ones keep their order).
+ * Sharing the GPU among fewer large requests shortens each of them, which
+ * is what sets the tail latency of a mixed stream. */
 typedef struct {
     size_t hbm_budget; /* bytes of device memory the server may hold */
     int32_t max_slots; /* requests in flight on the device (1..64) */
     int32_t slices;    /* moduli for split-class requests */
     const char *tune_table; /* FP64 tile shapes (NULL: built-in) */
+    double big_flops;  /* m n^2 from which a request counts as large */
+    int32_t max_big;   /* large requests in flight at once (0: no cap) */
+    int32_t reserved;  /* 0 */
 } uvt_serve_config;
 
 typedef struct {

@@ -144,3 +144,46 @@ def test_serve_device_operands(uvt, gpu, oracle):
         n, m = B.shape
         err = rel_fro(dC.download((n, m)), oracle.trmm(A, B))
         assert err <= (1e-13 if prec == 0 else 1e-12), (n, m, prec, err)
+
+
+def test_serve_large_request_cap(uvt, gpu, oracle):
+    """With max_big = 1, requests of m n^2 >= big_flops run one at a time
+    (their admit..done intervals do not overlap), a smaller request queued
+    behind a held-back large one is admitted before it, and every result is
+    bit for bit the same request served alone."""
+    # the first one takes about a millisecond, far longer than the submissions
+    reqs = [(4000, 1000, 0, 7001), (1400, 700, 1, 7002), (300, 50, 0, 7003), (1600, 500, 0, 7004),
+            (200, 40, 1, 7005), (1300, 800, 1, 7006), (250, 30, 0, 7007)]
+    big = 5e8  # the four large ones have m n^2 >= 1.28e9, the small ones <= 4.5e6
+    srv = uvt.Server(gpu, 8 * GiB, max_slots=8, slices=13, big_flops=big, max_big=1)
+    outs = [np.full((n, m), np.nan) for n, m, _, _ in reqs]
+    try:
+        # all submitted before the scheduler can retire the first large one
+        for k, (n, m, prec, seed) in enumerate(reqs):
+            srv.submit_seeded(n, m, prec, seed, spread=1, C=outs[k], tag=k)
+        srv.drain()
+        done = {d["tag"]: d for d in srv.poll()}
+    finally:
+        srv.close()
+    assert sorted(done) == list(range(len(reqs))) and all(d["status"] == uvt.OK for d in done.values())
+    large = sorted((done[k] for k, (n, m, _, _) in enumerate(reqs) if m * n * n >= big),
+                   key=lambda d: d["t_admit"])
+    assert len(large) == 4
+    assert [d["tag"] for d in large] == [0, 1, 3, 5]  # large ones keep their order
+    for a, b in zip(large, large[1:]):
+        assert b["t_admit"] >= a["t_done"], (a, b)
+    assert done[2]["t_admit"] < done[1]["t_admit"]  # the small one passed the held-back large one
+    alone, _ = serve(uvt, gpu, reqs, 8 * GiB, 1)
+    for k, (n, m, prec, seed) in enumerate(reqs):
+        assert np.array_equal(alone[k], outs[k]), (k, n, m)
+    for k in (0, 2):
+        n, m, prec, seed = reqs[k]
+        ref = oracle.trmm(oracle.seeded_A(n, seed, 1), oracle.seeded_B(n, m, seed, 1))
+        assert rel_fro(outs[k], ref) <= 1e-13
+
+
+def test_serve_config_checks(uvt, gpu):
+    for kw in ({"max_big": -1}, {"big_flops": -1.0}):
+        with pytest.raises(uvt.UvtError) as e:
+            uvt.Server(gpu, 1 << 30, max_slots=2, **kw)
+        assert e.value.status == uvt.ERR_ARG

@@ -43,7 +43,8 @@ class Problem(_c.Structure):
 
 
 class ServeConfig(_c.Structure):
-    _fields_ = [("hbm_budget", sz), ("max_slots", i32), ("slices", i32), ("tune_table", _c.c_char_p)]
+    _fields_ = [("hbm_budget", sz), ("max_slots", i32), ("slices", i32), ("tune_table", _c.c_char_p),
+                ("big_flops", f64), ("max_big", i32), ("reserved", i32)]
 
 
 class Request(_c.Structure):
@@ -382,7 +383,7 @@ class Batch:
 
 def serve_request_bytes(n, m, prec, slices=13):
     """Arena bytes one request of this size and class occupies while in flight."""
-    cfg = ServeConfig(1 << 40, 1, slices, None)
+    cfg = ServeConfig(1 << 40, 1, slices, None, 0.0, 0, 0)
     b = sz()
     check(lib().uvt_serve_request_bytes(_c.byref(cfg), n, m, prec, _c.byref(b)), "request_bytes")
     return b.value
@@ -391,9 +392,11 @@ def serve_request_bytes(n, m, prec, slices=13):
 class Server:
     """Continuous-batching TRMM server (see uvt_serve_create)."""
 
-    def __init__(self, handle, hbm_budget, max_slots=8, slices=13, table=None):
+    def __init__(self, handle, hbm_budget, max_slots=8, slices=13, table=None, big_flops=0.0,
+                 max_big=0):
         self.h = handle
-        cfg = ServeConfig(hbm_budget, max_slots, slices, table.encode() if table else None)
+        cfg = ServeConfig(hbm_budget, max_slots, slices, table.encode() if table else None,
+                          big_flops, max_big, 0)
         p = vp()
         check(lib().uvt_serve_create(handle.ptr, _c.byref(cfg), _c.byref(p)), "serve_create")
         self.ptr = p

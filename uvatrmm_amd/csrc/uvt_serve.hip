@@ -7,13 +7,15 @@
 // One scheduler thread owns every HIP call of the server:
 //   loop: collect finished slots (event query) -> free their arena blocks ->
 //         admit pending requests in FIFO order while a slot is free and the
-//         head request's operands fit in the arena -> launch them (inputs,
-//         product, optional copy-back) on the slot's stream -> sleep until a
-//         submission or ~50 us.
+//         next request's operands fit in the arena (a large request held back
+//         by the large-request cap lets later ones pass) -> launch them
+//         (inputs, product, optional copy-back) on the slot's stream -> sleep
+//         until a submission or ~50 us.
 // A request's kernels depend only on its own (n, m, class), so its result is
 // the same alone and in the mix.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -74,6 +76,7 @@ struct Pending {
 
 struct Slot {
     bool busy = false;
+    bool big = false;  // counts against the large-request cap
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
     Pending p{};
@@ -100,6 +103,8 @@ struct uvt_server_s {
     bool stop = false;
     uint64_t next_ticket = 1;
     int inflight = 0;
+    int big_inflight = 0;
+    int hwq = 4;  // hardware queues HIP multiplexes the slot streams onto
     // statistics
     int64_t submitted = 0, completed = 0, rejected = 0;
     std::vector<double> latencies;
@@ -184,6 +189,33 @@ static uvt_status launch(uvt_server s, Slot& sl)
     return UVT_OK;
 }
 
+static bool is_big(const uvt_serve_config& cfg, const uvt_request& r)
+{
+    return cfg.max_big > 0 && (double)r.m * (double)r.n * (double)r.n >= cfg.big_flops;
+}
+
+// A free slot on the least busy hardware queue.  HIP spreads streams over
+// its hardware queues in turn (GPU_MAX_HW_QUEUES of them, 4 by default), so
+// slot i shares a queue with slots i +- hwq; a kernel queued behind another
+// request's on the same queue waits for all of it.  Ties go to the lowest
+// slot.
+static Slot* pick_slot(uvt_server s)
+{
+    const int ns = (int)s->slots.size();
+    Slot* best = nullptr;
+    int best_busy = 0;
+    for (int i = 0; i < ns; ++i) {
+        if (s->slots[i].busy) continue;
+        int busy = 0;
+        for (int j = i % s->hwq; j < ns; j += s->hwq) busy += s->slots[j].busy ? 1 : 0;
+        if (!best || busy < best_busy) {
+            best = &s->slots[i];
+            best_busy = busy;
+        }
+    }
+    return best;
+}
+
 static void scheduler(uvt_server s)
 {
     (void)hipSetDevice(s->h->device);
@@ -228,32 +260,37 @@ static void scheduler(uvt_server s)
             }
             s->last_done = std::max(s->last_done, t);
             sl.busy = false;
+            if (sl.big) s->big_inflight--;
+            sl.big = false;
             s->inflight--;
             progress = true;
         }
         if (progress) s->cv_drain.notify_all();
-        // 2. admission, FIFO
-        while (!s->pending.empty()) {
-            Slot* free_slot = nullptr;
-            for (Slot& sl : s->slots)
-                if (!sl.busy) {
-                    free_slot = &sl;
-                    break;
-                }
+        // 2. admission: FIFO, except that a large request held back by the
+        // cap lets later requests pass (large ones keep their order); a
+        // request that does not fit in the arena stops admission, so memory
+        // is granted in arrival order
+        for (;;) {
+            auto it = s->pending.begin();
+            while (it != s->pending.end() && is_big(s->cfg, it->r) && s->big_inflight >= s->cfg.max_big)
+                ++it;
+            if (it == s->pending.end()) break;
+            Slot* free_slot = pick_slot(s);
             if (!free_slot) break;
-            const Pending& head = s->pending.front();
             // one contiguous block per request; none free yet: wait for
             // completions (with nothing in flight the arena is one free block,
             // and every queued request fits the budget)
-            const size_t off = s->arena.alloc(head.bytes);
+            const size_t off = s->arena.alloc(it->bytes);
             if (off == SIZE_MAX) break;
             free_slot->busy = true;
+            free_slot->big = is_big(s->cfg, it->r);
             free_slot->off = off;
-            free_slot->p = head;
+            free_slot->p = *it;
             free_slot->t_admit = s->now();
             free_slot->status = UVT_OK;
-            s->pending.pop_front();
+            s->pending.erase(it);  // before unlocking: submissions may grow the queue meanwhile
             s->inflight++;
+            if (free_slot->big) s->big_inflight++;
             s->peak_inflight = std::max(s->peak_inflight, s->inflight);
             // the arena and the slot belong to this thread: launch unlocked, so
             // submit and poll never wait behind copies or kernel launches
@@ -278,7 +315,8 @@ static void scheduler(uvt_server s)
 extern "C" uvt_status uvt_serve_create(uvt_handle h, const uvt_serve_config* cfg, uvt_server* out)
 {
     if (!h || !cfg || !out || cfg->max_slots < 1 || cfg->max_slots > 64 || cfg->slices < 2 ||
-        cfg->slices > 14 || cfg->hbm_budget < 4096)
+        cfg->slices > 14 || cfg->hbm_budget < 4096 || !(cfg->big_flops >= 0.0) || cfg->max_big < 0 ||
+        cfg->reserved != 0)
         return UVT_ERR_ARG;
     std::vector<uvt_tune_entry> tune;
     if (cfg->tune_table && !uvt_tune_load(cfg->tune_table, tune)) return UVT_ERR_ARG;
@@ -290,6 +328,7 @@ extern "C" uvt_status uvt_serve_create(uvt_handle h, const uvt_serve_config* cfg
     s->cfg.tune_table = nullptr;  // the caller's string need not outlive create
     s->tune = tune;
     s->t0 = std::chrono::steady_clock::now();
+    if (const char* q = getenv("GPU_MAX_HW_QUEUES")) s->hwq = std::max(1, atoi(q));
     if (!s->arena.init(cfg->hbm_budget)) {
         delete s;
         return UVT_ERR_NOMEM;
