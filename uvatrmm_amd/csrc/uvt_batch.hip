@@ -12,9 +12,9 @@
 //   -> join -> the split reconstruction, which adds the integer box part to
 //              the diagonal blocks the FP64 queue stored.
 // The FP64 queue holds every tile of the FP64 problems and the diagonal-block
-// tiles of the split problems.  Split-class problems too small for the
-// integer box to pay off (n < 768 or m < 64) run wholly in FP64, which meets
-// their tolerance.
+// tiles of the split problems.  Split-class problems for which the int8 path
+// is not faster in a batch (uvt_split_faster_in_batch) run wholly in FP64,
+// which meets their tolerance.
 // Every tile and every split stage works on one problem with a fixed
 // k-order, so a problem's result is the same alone or in any batch.
 #include <hip/hip_runtime.h>
