@@ -77,7 +77,7 @@ struct Desc {
     int* alpha;
     int* beta;
     double* part;
-    int8_t* Ar;               // [mod][pad_tile(n)/32][kmax/32] fragments (see k_resid_a)
+    int8_t* Ar;               // [mod][pad_tile(n)/16 row blocks of a_block_stride bytes] (see k_resid_a)
     int8_t* Br;               // [mod][pad_tile(m)][kmax] (B transposed)
     uint8_t* R;               // [mod][n][m]
     double target;
