@@ -23,7 +23,7 @@ void k_trmm_tiles_f64(const uvt_tile* __restrict__ tiles, const double* __restri
                       int64_t ldc, int64_t m)
 {
     using W = TileWaves<TM, TN>;
-    __shared__ double smem[F64Cfg<TM, TN, W::M, W::N>::SMEM_DOUBLES];
+    __shared__ __attribute__((aligned(16))) double smem[F64Cfg<TM, TN, W::M, W::N>::SMEM_DOUBLES];
     const uvt_tile t = tiles[blockIdx.x];
     f64_tile<TM, TN, VEC, W::M, W::N>(t.r0, t.rows, t.c0, t.k_lo, t.k_hi, t.flags, gbl(A), lda,
                                       gbl(B), ldb, gbl(C), ldc, m, smem);
@@ -39,7 +39,7 @@ void k_trmm_trapezoid_f64(int64_t n, int64_t m, int nrt, int nct, const double* 
                           double* __restrict__ C, int64_t ldc)
 {
     using W = TileWaves<TM, TN>;
-    __shared__ double smem[F64Cfg<TM, TN, W::M, W::N>::SMEM_DOUBLES];
+    __shared__ __attribute__((aligned(16))) double smem[F64Cfg<TM, TN, W::M, W::N>::SMEM_DOUBLES];
     const int rt = nrt - 1 - (int)(blockIdx.x / nct), ct = (int)(blockIdx.x % nct);
     const int r0 = rt * TM;
     const int rows = (int)(n - r0 < TM ? n - r0 : TM);

@@ -59,7 +59,7 @@ void k_batch_f64(const BatchTile* __restrict__ tiles, int ntiles, const BatchPro
                  int* __restrict__ counter)
 {
     using W = TileWaves<TM, TN>;
-    __shared__ double smem[F64Cfg<TM, TN, W::M, W::N>::SMEM_DOUBLES];
+    __shared__ __attribute__((aligned(16))) double smem[F64Cfg<TM, TN, W::M, W::N>::SMEM_DOUBLES];
     __shared__ int next;
     for (;;) {
         if (threadIdx.x == 0) next = atomicAdd(counter, 1);

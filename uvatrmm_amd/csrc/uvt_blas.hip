@@ -116,7 +116,7 @@ void k_gemm_f64(int64_t M, int64_t N, int K, int nct, const double* __restrict__
                 const double* __restrict__ B, int64_t ldb, double* __restrict__ C, int64_t ldc)
 {
     using W = TileWaves<TM, TN>;
-    __shared__ double smem[F64Cfg<TM, TN, W::M, W::N>::SMEM_DOUBLES];
+    __shared__ __attribute__((aligned(16))) double smem[F64Cfg<TM, TN, W::M, W::N>::SMEM_DOUBLES];
     const int rt = (int)(blockIdx.x / nct), ct = (int)(blockIdx.x % nct);
     const int r0 = rt * TM;
     const int rows = (int)(M - r0 < TM ? M - r0 : TM);

@@ -468,7 +468,7 @@ __global__ __launch_bounds__(512, 1) void k_gemm_i8(Desc one, const Desc* table,
 // the 256-row tile holding r0 (lower mask), written (not accumulated).
 __global__ __launch_bounds__(512, 4) void k_diag_f64(Desc one, const Desc* table, const int* lut, int nprob)
 {
-    __shared__ double smem[F64Cfg<128, 128, 4, 2>::SMEM_DOUBLES];
+    __shared__ __attribute__((aligned(16))) double smem[F64Cfg<128, 128, 4, 2>::SMEM_DOUBLES];
     const Desc* dp = nprob == 1 ? nullptr : &table[lut[blockIdx.x]];
 #define UVT_FIELD(f) (dp ? dp->f : one.f)
     const int64_t wb = (int64_t)blockIdx.x - UVT_FIELD(start[K_DIAG]);
