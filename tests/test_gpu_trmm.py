@@ -51,6 +51,34 @@ def test_small_exact_cases(gpu, uvt):
             np.testing.assert_array_equal(C, np.cumsum(B, axis=0))
 
 
+INT_CASES = [  # (n, m, p, mapping, shape, tile the plan picks)
+    (700, 40, 3, "reg", "boxes", (128, 32)),
+    (700, 40, 1, "reg", "trapezoid", (128, 32)),
+    (300, 200, 5, "balanced", "boxes", (64, 64)),
+    (1001, 333, 1, "balanced", "trapezoid", (64, 64)),
+    (4096, 2048, 7, "balanced", "boxes", (128, 128)),
+    (4099, 2050, 1, "reg", "trapezoid", (128, 128)),
+]
+
+
+@pytest.mark.parametrize("n,m,p,mp,shape,tile", INT_CASES)
+def test_integer_inputs_exact(gpu, uvt, n, m, p, mp, shape, tile):
+    """Small-integer A and B keep every partial sum exact in FP64, so the
+    result must equal tril(A) @ B bit for bit, whatever order the MFMA
+    steps add in.  A carries nonzero values above the diagonal, which must
+    not reach C.  Covers each tile shape, odd k starts (the boxes shape's
+    triangles) and partial edge tiles; a wrong pairing of A and B values
+    inside a k panel, or a mask off by one, changes integers."""
+    plan = uvt.Plan(n, m, p, mp, shape)
+    assert plan.tile_shape() == tile
+    plan.close()
+    rng = np.random.default_rng(n * 7919 + m)
+    A = rng.integers(-4, 5, size=(n, n)).astype(np.float64)
+    B = rng.integers(-4, 5, size=(n, m)).astype(np.float64)
+    C = run_trmm(gpu, uvt, A, B, p, mp, shape)
+    np.testing.assert_array_equal(C, np.tril(A) @ B)
+
+
 CASES = [
     (2048, 2048, 4, "reg", "boxes"),  # the CPU-runnable reference configuration
     (2048, 2048, 4, "reg", "trapezoid"),
