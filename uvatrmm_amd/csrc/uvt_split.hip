@@ -418,8 +418,13 @@ __global__ __launch_bounds__(512, 1) void k_gemm_i8(Desc one, const Desc* table,
     const int gr = UVT_FIELD(gr), gc = 32 / gr;
     const int ngc = (nct + gc - 1) / gc, per_rg = ngc * mods.n;
     const int rg = g / per_rg, grem = g % per_rg;
-    const int l = grem / ngc, ct = (grem % ngc) * gc + t % gc;
-    const int rt = nrt - 1 - (rg * gr + t / gc);
+    // Uniform over the workgroup by construction; readfirstlane tells the
+    // compiler, so the depth and the panel bases stay in scalar registers
+    // (the divisions above otherwise leave them in vector registers, and
+    // every k step's clamp and addresses become per-lane arithmetic).
+    const int l = __builtin_amdgcn_readfirstlane(grem / ngc);
+    const int ct = __builtin_amdgcn_readfirstlane((grem % ngc) * gc + t % gc);
+    const int rt = __builtin_amdgcn_readfirstlane(nrt - 1 - (rg * gr + t / gc));
     if (rt < 1 || ct >= nct) return;  // padding of the last groups (uniform per block)
     const int64_t r0 = (int64_t)rt * kTileI, c0 = (int64_t)ct * kTileI;
     const int64_t n = UVT_FIELD(n), m = UVT_FIELD(m), kmax = UVT_FIELD(kmax);
