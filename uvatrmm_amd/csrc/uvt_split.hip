@@ -259,48 +259,60 @@ __global__ __launch_bounds__(256) void k_resid_a(Desc one, const Desc* table, co
     }
 }
 
-// Br[l][j][k] (B transposed) for k < kmax: 64 x 64 tiles through LDS.
+// Br[l][j][k] (B transposed) for k < kmax: 128 (k) x 64 (j) tiles through LDS.
 // Thread (j = t%64, kq = t/64) converts k = 4 (kq + 4p) + e of column j
-// (p, e < 4), so it packs whole 4-byte words of the transposed tile; the words
-// go to LDS k-word-major (consecutive lanes, consecutive banks) and come back
-// as 16-byte row pieces.  Rows m .. pad_tile(m) are zeros.
+// (p < 8, e < 4), so it packs whole 4-byte words of the transposed tile; the
+// words go to LDS k-word-major (consecutive lanes, consecutive banks) and come
+// back as 16-byte pieces, 8 lanes to each 128-byte row piece of Br (whole
+// lines).  Two LDS buffers, so one barrier per modulus.  Rows m .. pad_tile(m)
+// are zeros.
 __global__ __launch_bounds__(256) void k_resid_b(Desc one, const Desc* table, const int* lut,
                                                  int nprob, Mods mods)
 {
-    __shared__ uint32_t t[16][64 + 4];  // [k word][column]
+    constexpr int kW = 32;          // 4-byte words per column of the tile (128 k)
+    constexpr int kPitch = 64 + 2;  // readback: 8 pieces x 8 rows of a wave on distinct banks
+    __shared__ uint32_t t[2][kW][kPitch];  // [buffer][k word][column]
     const Desc d = fetch(one, table, lut, nprob);
     const int64_t wb = (int64_t)blockIdx.x - d.start[K_RB];
-    const int64_t nkx = d.kmax / 64;
-    const int64_t k0 = (wb % nkx) * 64, j0 = (wb / nkx) * 64;
+    const int64_t nkx = d.kmax / 128;  // kmax is a multiple of 256
+    const int64_t k0 = (wb % nkx) * 128, j0 = (wb / nkx) * 64;
     const int jj = threadIdx.x & 63, kq = threadIdx.x >> 6;
     const int64_t j = j0 + jj;
     const bool jv = j < d.m;
     const int be = jv ? gbl(d.beta)[j] : 0;
-    double v[16];
+    double v[kW];
 #pragma unroll
-    for (int p = 0; p < 4; ++p)
+    for (int p = 0; p < kW / 4; ++p)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const int64_t k = k0 + 4 * (kq + 4 * p) + e;  // < kmax: kmax is a multiple of 64
+            const int64_t k = k0 + 4 * (kq + 4 * p) + e;  // < kmax
             v[p * 4 + e] = jv ? rint(ldexp(gbl(d.B)[k * d.ldb + j], be)) : 0.0;
         }
-    const int wr = threadIdx.x >> 2, ws = threadIdx.x & 3;
+    const int pr = threadIdx.x >> 3, ps = threadIdx.x & 7;  // readback rows pr, pr + 32; piece ps
     const int64_t mod_stride = pad_tile(d.m) * d.kmax;
     for (int l = 0; l < mods.n; ++l) {
+        uint32_t(*tb)[kPitch] = t[l & 1];
 #pragma unroll
-        for (int p = 0; p < 4; ++p) {
+        for (int p = 0; p < kW / 4; ++p) {
             uint32_t x = 0;
 #pragma unroll
             for (int e = 0; e < 4; ++e)
                 x |= (uint32_t)(uint8_t)(int8_t)sym_residue(v[p * 4 + e], mods.m[l], mods.invd[l])
                      << (8 * e);
-            t[kq + 4 * p][jj] = x;
+            tb[kq + 4 * p][jj] = x;
         }
+        // One barrier per modulus: buffer l & 1 is written again only at
+        // l + 2, after every thread has passed the barrier of l + 1 and so
+        // finished reading it here.
         __syncthreads();
-        const v4i x = {(int)t[4 * ws][wr], (int)t[4 * ws + 1][wr], (int)t[4 * ws + 2][wr],
-                       (int)t[4 * ws + 3][wr]};
-        *reinterpret_cast<UVT_GLOBAL v4i*>(gbl(d.Br) + l * mod_stride + (j0 + wr) * d.kmax + k0 + 16 * ws) = x;
-        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int row = pr + 32 * h;
+            const v4i x = {(int)tb[4 * ps][row], (int)tb[4 * ps + 1][row], (int)tb[4 * ps + 2][row],
+                           (int)tb[4 * ps + 3][row]};
+            *reinterpret_cast<UVT_GLOBAL v4i*>(gbl(d.Br) + l * mod_stride + (j0 + row) * d.kmax + k0 +
+                                               16 * ps) = x;
+        }
     }
 }
 
@@ -750,7 +762,7 @@ void describe(const uvt_split_problem* probs, int count, int nmod, char* work, D
             d.count[K_BPART] = (int32_t)(((q.m + 255) / 256) * d.chunks);
             d.count[K_BETA] = (int32_t)((q.m + 255) / 256);
             d.count[K_RA] = (int32_t)((pad_tile(q.n) - kTileI) / 32 * d.gxa);
-            d.count[K_RB] = (int32_t)((d.kmax / 64) * (pad_tile(q.m) / 64));
+            d.count[K_RB] = (int32_t)((d.kmax / 128) * (pad_tile(q.m) / 64));
             d.gr = group_rows(d.nrt - 1, d.nct);
             const int64_t groups =
                 (int64_t)((d.nrt - 1 + d.gr - 1) / d.gr) * ((d.nct + 32 / d.gr - 1) / (32 / d.gr)) * nmod;
