@@ -151,8 +151,11 @@ def test_serve_large_request_cap(uvt, gpu, oracle):
     (their admit..done intervals do not overlap), a smaller request queued
     behind a held-back large one is admitted before it, and every result is
     bit for bit the same request served alone."""
-    # the first one takes about a millisecond, far longer than the submissions
-    reqs = [(4000, 1000, 0, 7001), (1400, 700, 1, 7002), (300, 50, 0, 7003), (1600, 500, 0, 7004),
+    # The first one keeps the GPU busy for about 4 ms: longer than the
+    # submissions and than the scheduler's own launch of it (about 1.4 ms
+    # has been seen for a launch on a stream's first use), so the cap is
+    # still holding request 1 back when the small ones are looked at.
+    reqs = [(8192, 4096, 0, 7001), (1400, 700, 1, 7002), (300, 50, 0, 7003), (1600, 500, 0, 7004),
             (200, 40, 1, 7005), (1300, 800, 1, 7006), (250, 30, 0, 7007)]
     big = 5e8  # the four large ones have m n^2 >= 1.28e9, the small ones <= 4.5e6
     srv = uvt.Server(gpu, 8 * GiB, max_slots=8, slices=13, big_flops=big, max_big=1)
@@ -166,17 +169,21 @@ def test_serve_large_request_cap(uvt, gpu, oracle):
     finally:
         srv.close()
     assert sorted(done) == list(range(len(reqs))) and all(d["status"] == uvt.OK for d in done.values())
+    times = {k: (round(d["t_submit"] * 1e3, 3), round(d["t_admit"] * 1e3, 3), round(d["t_done"] * 1e3, 3))
+             for k, d in done.items()}  # ms: submit, admit, done
+    print("[gpu] serve cap times", times, flush=True)
     large = sorted((done[k] for k, (n, m, _, _) in enumerate(reqs) if m * n * n >= big),
                    key=lambda d: d["t_admit"])
     assert len(large) == 4
-    assert [d["tag"] for d in large] == [0, 1, 3, 5]  # large ones keep their order
+    assert [d["tag"] for d in large] == [0, 1, 3, 5], times  # large ones keep their order
     for a, b in zip(large, large[1:]):
-        assert b["t_admit"] >= a["t_done"], (a, b)
-    assert done[2]["t_admit"] < done[1]["t_admit"]  # the small one passed the held-back large one
+        assert b["t_admit"] >= a["t_done"], times
+    assert done[2]["t_submit"] < done[0]["t_done"], times  # the scenario: 0 still running
+    assert done[2]["t_admit"] < done[1]["t_admit"], times  # the small one passed the held-back large one
     alone, _ = serve(uvt, gpu, reqs, 8 * GiB, 1)
     for k, (n, m, prec, seed) in enumerate(reqs):
         assert np.array_equal(alone[k], outs[k]), (k, n, m)
-    for k in (0, 2):
+    for k in (2, 6):
         n, m, prec, seed = reqs[k]
         ref = oracle.trmm(oracle.seeded_A(n, seed, 1), oracle.seeded_B(n, m, seed, 1))
         assert rel_fro(outs[k], ref) <= 1e-13
