@@ -31,11 +31,11 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
 # HBM bytes per launch of the FP64 tile kernel at N=32768, M=8192 (balanced,
 # trapezoid, 16 bands), from two rocprofv3 counter runs of this script:
-# 2.0 x FETCH_SIZE (92,074,150 KiB; the factor is what FETCH_SIZE reads for
+# 2.0 x FETCH_SIZE (93,084,523 KiB; the factor is what FETCH_SIZE reads for
 # k_row_abs_sums streaming exactly 2 GiB in 128-byte requests, the request
 # size of the tile kernel's 128-byte row segments) + WRITE_SIZE (2,129,792
 # KiB).  Kept per workload; other workloads report null.
-MEASURED_TRAFFIC = {(32768, 8192, 16, "balanced", "trapezoid"): 2.0 * 92074150 * 1024 + 2129792 * 1024}
+MEASURED_TRAFFIC = {(32768, 8192, 16, "balanced", "trapezoid"): 2.0 * 93084523 * 1024 + 2129792 * 1024}
 
 
 def log(*a):
