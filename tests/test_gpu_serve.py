@@ -152,9 +152,9 @@ def test_serve_large_request_cap(uvt, gpu, oracle):
     behind a held-back large one is admitted before it, and every result is
     bit for bit the same request served alone."""
     # The first one keeps the GPU busy for about 4 ms: longer than the
-    # submissions and than the scheduler's own launch of it (about 1.4 ms
-    # has been seen for a launch on a stream's first use), so the cap is
-    # still holding request 1 back when the small ones are looked at.
+    # submissions and than the scheduler's own launch of it (about 1.5 ms
+    # has been seen for the first launch of a kernel in a process), so the
+    # cap is still holding request 1 back when the small ones are looked at.
     reqs = [(8192, 4096, 0, 7001), (1400, 700, 1, 7002), (300, 50, 0, 7003), (1600, 500, 0, 7004),
             (200, 40, 1, 7005), (1300, 800, 1, 7006), (250, 30, 0, 7007)]
     big = 5e8  # the four large ones have m n^2 >= 1.28e9, the small ones <= 4.5e6
