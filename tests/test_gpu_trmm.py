@@ -188,16 +188,22 @@ def test_inf_norm_and_row_sums(gpu, uvt, oracle):
     out.free()
 
 
-def test_odd_leading_dimension_and_upper_garbage(gpu, uvt, oracle):
-    """lda odd (unaligned rows: scalar-load path) and junk above the diagonal."""
+@pytest.mark.parametrize("pad", [1, 2])
+def test_odd_leading_dimension_and_upper_garbage(gpu, uvt, oracle, pad):
+    """Junk above the diagonal (huge values, NaN) and in the row padding must
+    never reach C: lda odd (unaligned rows, 8-byte loads) or even (16-byte
+    loads, where a diagonal panel loads both elements of a pair and zeroes
+    the one above the diagonal)."""
     n, m = 700, 90
     A = oracle.seeded_A(n, 3)
     B = oracle.seeded_B(n, m, 3)
-    Abig = np.full((n, n + 1), np.nan)
-    Abig[:, :n] = A + np.triu(np.full((n, n), 1e300), 1)
+    upper = np.triu(np.ones((n, n), dtype=bool), 1)
+    Abig = np.full((n, n + pad), np.nan)
+    Abig[:, :n] = np.where(upper, 1e300, A)
+    Abig[:, :n][upper & (np.arange(n)[None, :] % 3 == 0)] = np.nan
     ref = oracle.trmm(A, B)
     for shape in ("boxes", "trapezoid"):
-        C = run_trmm(gpu, uvt, Abig, B, 3, "reg", shape, lda=n + 1)
+        C = run_trmm(gpu, uvt, Abig, B, 3, "reg", shape, lda=n + pad)
         assert rel_fro(C, ref) <= TOL
 
 
