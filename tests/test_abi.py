@@ -30,6 +30,25 @@ def test_library_exports_declared_symbols(uvt, header):
     assert not missing, missing
 
 
+@pytest.mark.parametrize("header", HEADERS)
+def test_library_exports_nothing_undeclared(uvt, header):
+    """The exported entry points are exactly the header's: library-internal
+    helpers stay hidden."""
+    path = uvt.LIB_PATH if header != "uvt_cblas.h" else uvt.CBLAS_PATH
+    try:
+        out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True,
+                             timeout=60, check=True).stdout
+    except (OSError, subprocess.CalledProcessError):
+        pytest.skip("nm not available")
+    exported = set()
+    for line in out.splitlines():
+        parts = line.split()
+        if len(parts) == 3 and parts[1] == "T" and re.match(r"(uvt_|cblas_)", parts[2]):
+            exported.add(parts[2])
+    extra = sorted(exported - set(declared(header)))
+    assert not extra, extra
+
+
 def test_version_and_status_strings(uvt):
     assert b"gfx950" in uvt.lib().uvt_version()
     assert uvt.lib().uvt_status_string(uvt.ERR_MAPPING).startswith(b"mapping")

@@ -11,10 +11,13 @@
 extern "C" {
 #endif
 
+/* Library-internal: linked across libuvt's own sources, not exported. */
+#define UVT_INTERNAL __attribute__((visibility("hidden")))
+
 /* FP64 tile kernel configurations (rows x columns of one workgroup tile). */
 enum { UVT_CFG_128x128 = 0, UVT_CFG_64x64 = 1, UVT_CFG_128x32 = 2, UVT_NUM_CFGS = 3 };
-int32_t uvt_cfg_tile_m(int32_t cfg);
-int32_t uvt_cfg_tile_n(int32_t cfg);
+UVT_INTERNAL int32_t uvt_cfg_tile_m(int32_t cfg);
+UVT_INTERNAL int32_t uvt_cfg_tile_n(int32_t cfg);
 
 struct uvt_plan_s {
     int64_t n, m;
@@ -34,12 +37,12 @@ struct uvt_plan_s {
 
 /* Planner entry used by the batch path: plan one problem with a given tile
  * configuration into caller-owned arrays (NULL out = count only). */
-int64_t uvt_plan_trapezoid_tiles(int64_t n, int64_t m, int32_t cfg, uvt_tile *out);
+UVT_INTERNAL int64_t uvt_plan_trapezoid_tiles(int64_t n, int64_t m, int32_t cfg, uvt_tile *out);
 /* Built-in tile-shape choice for an n x n by n x m product. */
-int32_t uvt_default_cfg(int64_t n, int64_t m);
+UVT_INTERNAL int32_t uvt_default_cfg(int64_t n, int64_t m);
 
 /* Releases device copies of a plan (defined on the HIP side). */
-void uvt_plan_release_device(uvt_plan plan);
+UVT_INTERNAL void uvt_plan_release_device(uvt_plan plan);
 
 #ifdef __cplusplus
 }
