@@ -161,6 +161,23 @@ uvt_status uvt_dtrmm(uvt_handle h, uvt_side side, uvt_uplo uplo, uvt_trans ta, u
                      int64_t m, int64_t n, double alpha, const double *dA, int64_t lda,
                      double *dB, int64_t ldb, void *stream);
 
+/* uvt_dtrmm that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * results (bit for bit) and errors as uvt_dtrmm, but its workspace is a fixed
+ * per-device scratch array of UVT_NOALLOC_SCRATCH_MB MiB inside the library
+ * (held by every device the library runs on until the process ends) instead
+ * of the handle's workspace: calls needing more (for a left-side lower,
+ * non-transposed, non-unit A: 8 * m * (n rounded up to even) bytes; other
+ * forms also pack A and/or B) return UVT_ERR_SIZE.  Every call on a device
+ * shares that device's array, so calls that overlap on one device (other
+ * streams, other handles, other threads) race: serialize them.  Nothing in
+ * the array needs to start at zero. */
+#define UVT_NOALLOC_SCRATCH_MB 128
+uvt_status uvt_dtrmm_noalloc(uvt_handle h, uvt_side side, uvt_uplo uplo, uvt_trans ta,
+                             uvt_diag diag, int64_t m, int64_t n, double alpha, const double *dA,
+                             int64_t lda, double *dB, int64_t ldb, void *stream);
+
 /* ------------------------------------------------------------------ */
 /* Batched TRMMs (no counterpart in the reference).                    */
 /* ------------------------------------------------------------------ */

@@ -155,6 +155,11 @@ extern "C" uvt_status uvt_create(int device, uvt_handle* out)
     h->device = device;
     h->cus = prop.multiProcessorCount;
     h->vram = prop.totalGlobalMem;
+    const uvt_status st = uvt_noalloc_setup(h);
+    if (st != UVT_OK) {
+        free(h);
+        return st;
+    }
     *out = h;
     return UVT_OK;
 }

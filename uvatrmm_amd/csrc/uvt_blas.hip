@@ -21,6 +21,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "uvt_f64_tile.hpp"
 #include "uvt_internal.h"
@@ -241,67 +242,157 @@ extern "C" uvt_status uvt_dgemm(uvt_handle h, uvt_trans ta, uvt_trans tb, int64_
     return UVT_OK;
 }
 
-extern "C" uvt_status uvt_dtrmm(uvt_handle h, uvt_side side, uvt_uplo uplo, uvt_trans ta,
-                                uvt_diag diag, int64_t m, int64_t n, double alpha,
-                                const double* dA, int64_t lda, double* dB, int64_t ldb,
-                                void* stream)
+// ---- dtrmm ---------------------------------------------------------------------
+
+// Scratch of uvt_dtrmm_noalloc: a fixed device array, one copy per device,
+// held until the process ends.  Host code never passes `scratch` itself to a
+// kernel (in host code it is only a stand-in for the device symbol); it uses
+// the address hipGetSymbolAddress returned for the handle's device, which
+// uvt_noalloc_setup stores in the handle.
+constexpr size_t kNoAllocScratchBytes = (size_t)UVT_NOALLOC_SCRATCH_MB << 20;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+namespace {
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices];  // per device, set once
+
+// One dtrmm call, reduced to R = tril(L) X (see the comment at the top of
+// this file): which operands are packed and the workspace bytes of each.
+struct DtrmmCall {
+    bool right, tA, rev, unit, tX, packL, packX;
+    int64_t nt, w, ll, lx, lr;
+    size_t l_bytes, x_bytes, r_bytes;
+};
+
+// Validates the arguments and sizes the workspace.  UVT_OK with
+// *nothing = true when B is empty (no work at all).
+uvt_status dtrmm_prepare(uvt_handle h, uvt_side side, uvt_uplo uplo, uvt_trans ta, uvt_diag diag,
+                         int64_t m, int64_t n, double alpha, const double* dA, int64_t lda,
+                         double* dB, int64_t ldb, DtrmmCall* c, bool* nothing)
 {
+    *nothing = false;
     if (!h || m < 0 || n < 0) return UVT_ERR_ARG;
     if ((side != UVT_LEFT && side != UVT_RIGHT) || (uplo != UVT_LOWER && uplo != UVT_UPPER) ||
         (ta != UVT_NO_TRANS && ta != UVT_TRANS) || (diag != UVT_NON_UNIT && diag != UVT_UNIT))
         return UVT_ERR_ARG;
-    const bool right = side == UVT_RIGHT;
-    const int64_t nt = right ? n : m;  // order of the triangle
-    const int64_t w = right ? m : n;   // columns of X
-    if (lda < std::max<int64_t>(1, nt) || ldb < std::max<int64_t>(1, n)) return UVT_ERR_ARG;
-    if (m == 0 || n == 0) return UVT_OK;
+    c->right = side == UVT_RIGHT;
+    c->nt = c->right ? n : m;  // order of the triangle
+    c->w = c->right ? m : n;   // columns of X
+    if (lda < std::max<int64_t>(1, c->nt) || ldb < std::max<int64_t>(1, n)) return UVT_ERR_ARG;
+    if (m == 0 || n == 0) {
+        *nothing = true;
+        return UVT_OK;
+    }
     if (!dB || (alpha != 0.0 && !dA)) return UVT_ERR_ARG;
     if (!sizes_ok(m, n, 1)) return UVT_ERR_SIZE;
-    UVT_HIP(hipSetDevice(h->device));
-    hipStream_t s = (hipStream_t)stream;
+    // T = op(A) (left) or op(A)^T (right), the nt x nt factor applied to X
+    c->tA = (ta == UVT_TRANS) != c->right;
+    const bool t_lower = (uplo == UVT_LOWER) != c->tA;
+    c->rev = !t_lower;  // upper T: reverse rows and columns
+    c->unit = diag == UVT_UNIT;
+    c->tX = c->right;   // X = B^T
+    c->packL = c->tA || c->rev || c->unit;
+    c->ll = even(c->nt);
+    c->lx = even(c->w);
+    c->lr = even(c->w);
+    c->l_bytes = c->packL ? up256((size_t)c->nt * c->ll * 8) : 0;
+    c->packX = c->tX || c->rev;  // else X is B itself (R is written elsewhere)
+    c->x_bytes = c->packX ? up256((size_t)c->nt * c->lx * 8) : 0;
+    c->r_bytes = up256((size_t)c->nt * c->lr * 8);
+    if (alpha == 0.0) {  // no workspace: B = 0, A not read
+        c->l_bytes = c->x_bytes = c->r_bytes = 0;
+    }
+    return UVT_OK;
+}
+
+size_t dtrmm_bytes(const DtrmmCall& c) { return c.l_bytes + c.x_bytes + c.r_bytes; }
+
+// Enqueues the call on stream s with its workspace at wv.
+uvt_status dtrmm_launch(const DtrmmCall& c, int64_t m, int64_t n, double alpha, const double* dA,
+                        int64_t lda, double* dB, int64_t ldb, void* wv, hipStream_t s)
+{
     if (alpha == 0.0) {  // B = 0, A not read
         axpby(m, n, 0.0, nullptr, 0, 0.0, dB, ldb, s);
         UVT_HIP(hipGetLastError());
         return UVT_OK;
     }
-    // T = op(A) (left) or op(A)^T (right), the nt x nt factor applied to X
-    const bool tA = (ta == UVT_TRANS) != right;
-    const bool t_lower = (uplo == UVT_LOWER) != tA;
-    const bool rev = !t_lower;  // upper T: reverse rows and columns
-    const bool unit = diag == UVT_UNIT;
-    const bool tX = right;      // X = B^T
-    const bool packL = tA || rev || unit;
-    const int64_t ll = even(nt), lx = even(w), lr = even(w);
-    const size_t l_bytes = packL ? up256((size_t)nt * ll * 8) : 0;
-    const bool packX = tX || rev;  // else X is B itself (R is written elsewhere)
-    const size_t x_bytes = packX ? up256((size_t)nt * lx * 8) : 0;
-    const size_t r_bytes = up256((size_t)nt * lr * 8);
-    void* wv = nullptr;
-    uvt_status st = uvt_handle_workspace(h, l_bytes + x_bytes + r_bytes, &wv);
-    if (st != UVT_OK) return st;
     char* wp = (char*)wv;
     const double* L = dA;
     int64_t la = lda;
-    if (packL) {
-        pack(dA, lda, (double*)wp, ll, nt, nt, tA, rev, rev, 1.0, true, unit, s);
+    if (c.packL) {
+        pack(dA, lda, (double*)wp, c.ll, c.nt, c.nt, c.tA, c.rev, c.rev, 1.0, true, c.unit, s);
         L = (const double*)wp;
-        la = ll;
-        wp += l_bytes;
+        la = c.ll;
+        wp += c.l_bytes;
     }
     const double* X = dB;
     int64_t lxx = ldb;
-    if (packX) {
-        pack(dB, ldb, (double*)wp, lx, nt, w, tX, rev, false, 1.0, false, false, s);
+    if (c.packX) {
+        pack(dB, ldb, (double*)wp, c.lx, c.nt, c.w, c.tX, c.rev, false, 1.0, false, false, s);
         X = (const double*)wp;
-        lxx = lx;
-        wp += x_bytes;
+        lxx = c.lx;
+        wp += c.x_bytes;
     }
     double* R = (double*)wp;
-    st = uvt_launch_trapezoid_f64(uvt_default_cfg(nt, w), nt, w, L, la, X, lxx, R, lr, s);
+    uvt_status st = uvt_launch_trapezoid_f64(uvt_default_cfg(c.nt, c.w), c.nt, c.w, L, la, X, lxx,
+                                             R, c.lr, s);
     if (st != UVT_OK) return st;
     // B <- alpha * unpack(R): B(i, j) = R(rmap(i), j) (left) or R(cmap(j), i) (right)
-    if (tX) pack(R, lr, dB, ldb, m, n, true, false, rev, alpha, false, false, s);
-    else pack(R, lr, dB, ldb, m, n, false, rev, false, alpha, false, false, s);
+    if (c.tX) pack(R, c.lr, dB, ldb, m, n, true, false, c.rev, alpha, false, false, s);
+    else pack(R, c.lr, dB, ldb, m, n, false, c.rev, false, alpha, false, false, s);
     UVT_HIP(hipGetLastError());
     return UVT_OK;
+}
+
+}  // namespace
+
+// Looks up this device's copy of `scratch` once per device (with the device
+// current, as uvt_create leaves it) and stores its address in the handle.
+uvt_status uvt_noalloc_setup(uvt_handle h)
+{
+    if (h->device < 0 || h->device >= kMaxDevices) return UVT_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[h->device]) {
+        void* p = nullptr;
+        UVT_HIP(hipSetDevice(h->device));
+        UVT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return UVT_ERR_HIP;
+        g_scratch_ptr[h->device] = p;
+    }
+    h->scratch = g_scratch_ptr[h->device];
+    return UVT_OK;
+}
+
+extern "C" uvt_status uvt_dtrmm(uvt_handle h, uvt_side side, uvt_uplo uplo, uvt_trans ta,
+                                uvt_diag diag, int64_t m, int64_t n, double alpha,
+                                const double* dA, int64_t lda, double* dB, int64_t ldb,
+                                void* stream)
+{
+    DtrmmCall c;
+    bool nothing = false;
+    uvt_status st = dtrmm_prepare(h, side, uplo, ta, diag, m, n, alpha, dA, lda, dB, ldb, &c, &nothing);
+    if (st != UVT_OK || nothing) return st;
+    UVT_HIP(hipSetDevice(h->device));
+    void* wv = nullptr;
+    if (dtrmm_bytes(c)) {
+        st = uvt_handle_workspace(h, dtrmm_bytes(c), &wv);
+        if (st != UVT_OK) return st;
+    }
+    return dtrmm_launch(c, m, n, alpha, dA, lda, dB, ldb, wv, (hipStream_t)stream);
+}
+
+extern "C" uvt_status uvt_dtrmm_noalloc(uvt_handle h, uvt_side side, uvt_uplo uplo, uvt_trans ta,
+                                        uvt_diag diag, int64_t m, int64_t n, double alpha,
+                                        const double* dA, int64_t lda, double* dB, int64_t ldb,
+                                        void* stream)
+{
+    DtrmmCall c;
+    bool nothing = false;
+    uvt_status st = dtrmm_prepare(h, side, uplo, ta, diag, m, n, alpha, dA, lda, dB, ldb, &c, &nothing);
+    if (st != UVT_OK || nothing) return st;
+    if (dtrmm_bytes(c) > kNoAllocScratchBytes) return UVT_ERR_SIZE;
+    if (!h->scratch) return UVT_ERR_STATE;
+    UVT_HIP(hipSetDevice(h->device));
+    return dtrmm_launch(c, m, n, alpha, dA, lda, dB, ldb, h->scratch, (hipStream_t)stream);
 }

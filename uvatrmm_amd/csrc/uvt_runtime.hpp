@@ -15,7 +15,15 @@ struct uvt_handle_s {
     size_t vram;
     void* work;        // grows on demand (norm row sums, BLAS-form packing)
     size_t work_bytes;
+    void* scratch;     // this device's fixed scratch array (uvt_dtrmm_noalloc)
 };
+
+constexpr int kMaxDevices = 64;
+
+// Stores the handle's device's address of the fixed scratch array of
+// uvt_dtrmm_noalloc in h->scratch (looked up once per device); called by
+// uvt_create with the device current.
+UVT_INTERNAL uvt_status uvt_noalloc_setup(uvt_handle h);
 
 // Returns UVT_ERR_HIP from the enclosing function when a HIP call fails,
 // clearing HIP's sticky last error so later calls start clean.
