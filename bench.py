@@ -11,10 +11,16 @@ U[-1,1) with exponents spread over 2^[-8,8] (seeded generator, on device).
 Prints one JSON line.  value = M*N^2 / (mean step time) in TFLOP/s.
 roofline: the dominant kernel (the tile kernel) against FP64 MFMA peak,
 timed with HIP events on the stream the kernel runs on.
+Beside it: the split-precision leg on the same inputs (time, and the error
+statement of each call), the 512-problem batch (routed by measured speed,
+and with its split half forced onto the int8 path), the reference's other
+two configurations, and the served stream.
 cpu_baseline: the reference's per-rank stage loop (cblas_dgemm + cblas_dtrmm
 with MKL, sequential, one thread per rank) replayed on the host cores for a
-bounded column sample of the same matrices; its C columns double as a parity
-check of the GPU result.
+bounded column sample of the same matrices, median of 5 runs, for this
+workload and the reference's N=2048 and N=16384 configurations.  The same
+leg computes sampled rows of the exact product (double-double sums) and
+reports the error of the FP64 and split results on them.
 """
 import argparse
 import ctypes
@@ -27,6 +33,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+
+from uvatrmm_amd import workloads  # noqa: E402  (seeded workload definitions)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
 # HBM bytes per launch of the FP64 tile kernel at N=32768, M=8192 (balanced,
@@ -42,64 +50,152 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(args, C_gpu_cols, cols_needed):
-    """Reference stage loop on host cores (MKL) for a column sample."""
+CPU_RUNS = 5
+
+
+def host_ranks():
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return max(1, min(avail, int(os.environ.get("OMP_NUM_THREADS", avail)), 16))
+
+
+def stage_loop_median(orc, n, m_total, bands, mapping, ranks, w, seed, spread, have_mkl):
+    """The reference stage loop on `ranks` host threads (one per rank, each
+    over its own w columns with sequential MKL), CPU_RUNS times; returns
+    (median seconds, all runs, C of the last run, the flops of one run)."""
+    A = np.empty((n, n))
+    orc.L.orc_fill_seeded_A(n, seed, spread, orc._p(A), n)
+    B = np.empty((n, m_total))
+    orc.L.orc_fill_seeded_B(n, m_total, seed, spread, orc._p(B), m_total)
+    Bs = np.ascontiguousarray(B[:, : ranks * w])
+    del B
+    runs, Cs = [], None
+    for _ in range(CPU_RUNS):
+        Cs = np.zeros((n, ranks * w))
+        secs = orc.L.orc_ranks_stage_loop(n, w, bands, 1 if mapping == "balanced" else 0, ranks,
+                                          orc._p(A), n, orc._p(Bs), ranks * w, orc._p(Cs),
+                                          ranks * w, 1 if have_mkl else 0)
+        if secs < 0:
+            raise RuntimeError("stage loop failed")
+        runs.append(secs)
+    return float(np.median(runs)), runs, Cs, float(n) * n * (ranks * w)
+
+
+def scalapack_leg(args, C_gpu_cols, ranks):
+    """The reference's second comparator, MKL ScaLAPACK pdtrmm on a 2-D
+    block-cyclic grid (nb = 256), through oracle/scalapack_trmm.x under
+    MPICH: this workload on a 1024-column sample of B (its norm checked
+    against the GPU's same columns), and the N=2048 (all columns) and
+    N=16384 (2048 columns) configurations; median of CPU_RUNS runs each."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "oracle", "scalapack_trmm.x")
+    mpiexec = "/opt/conda/bin/mpiexec"
+    if not (os.path.exists(exe) and os.path.exists(mpiexec)):
+        return {"skipped": "scalapack_trmm.x or mpiexec missing"}
+    grids = {16: (4, 4), 8: (2, 4), 4: (2, 2), 2: (1, 2), 1: (1, 1)}
+    p = max(k for k in grids if k <= ranks)
+    out = []
+    for n, m, cmp_gpu in ((args.n, 1024, True), (2048, 2048, False), (16384, 2048, False)):
+        pr, pc = grids[p]
+        cmd = [mpiexec, "-n", str(p), exe, str(n), str(m), "256", str(pr), str(pc), str(CPU_RUNS),
+               str(args.seed), str(args.spread)]
+        env = dict(os.environ, OMP_NUM_THREADS="1", MKL_NUM_THREADS="1")
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=600, env=env)
+        if r.returncode != 0:
+            out.append({"workload": f"N={n} M={m}", "failed": (r.stderr or r.stdout)[-400:]})
+            continue
+        f = r.stdout.split()
+        med, fro = float(f[6]), float(f[9])
+        rec = {"workload": f"pdtrmm L,L,N,N N={n} M={m}", "value": round(float(m) * n * n / med * 1e-12, 4),
+               "unit": "TFLOP/s", "cores": p, "grid": f"{pr}x{pc}", "nb": 256, "kind": "port",
+               "seconds": med, "min_s": float(f[7]), "max_s": float(f[8]), "runs": CPU_RUNS}
+        if cmp_gpu and C_gpu_cols is not None and C_gpu_cols.shape[1] >= m:
+            g = float(np.linalg.norm(C_gpu_cols[:, :m]))
+            rec["gpu_vs_scalapack_norm_rel_diff"] = abs(g - fro) / fro
+        out.append(rec)
+        log(f"scalapack pdtrmm N={n} M={m}: {med:.3f} s, {rec['value']} TFLOP/s on {p} ranks")
+    return out
+
+
+def cpu_baseline(args, C_gpu_cols, rows_gpu):
+    """Reference stage loop on host cores (MKL) for column samples of this
+    workload and of the reference's N=2048 / N=16384 configurations, and the
+    exact sampled rows that check the GPU results."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from conftest import Oracle  # the oracle library: CPU comparator and checker
 
     orc = Oracle()
     ver = ctypes.create_string_buffer(256)
     have_mkl = bool(orc.L.orc_blas_available(ver, 256))
-    try:
-        avail = len(os.sched_getaffinity(0))
-    except AttributeError:
-        avail = os.cpu_count() or 1
-    ranks = max(1, min(avail, int(os.environ.get("OMP_NUM_THREADS", avail)), 16))
+    ranks = host_ranks()
     n, m = args.n, args.m
     w = args.cpu_cols_per_rank if have_mkl else max(2, args.cpu_cols_per_rank // 8)
     w = min(w, m // ranks)
-    log(f"cpu baseline: {ranks} ranks x {w} columns, mkl={have_mkl}")
-    t0 = time.time()
-    A = np.empty((n, n))
-    orc.L.orc_fill_seeded_A(n, args.seed, args.spread, orc._p(A), n)
-    B = np.empty((n, m))
-    orc.L.orc_fill_seeded_B(n, m, args.seed, args.spread, orc._p(B), m)
-    Bs = np.ascontiguousarray(B[:, : ranks * w])
-    del B
-    Cs = np.zeros((n, ranks * w))
-    log(f"host inputs ready in {time.time() - t0:.1f}s")
-    secs = orc.L.orc_ranks_stage_loop(n, w, args.bands, 1 if args.map == "balanced" else 0, ranks,
-                                      orc._p(A), n, orc._p(Bs), ranks * w, orc._p(Cs), ranks * w,
-                                      1 if have_mkl else 0)
-    flops = float(n) * n * (ranks * w)
+    lib = f"MKL {ver.value.decode()[:40]} sequential" if have_mkl else "C loops"
+    log(f"cpu baseline: {ranks} ranks x {w} columns, mkl={have_mkl}, median of {CPU_RUNS}")
+    med, runs, Cs, flops = stage_loop_median(orc, n, m, args.bands, args.map, ranks, w, args.seed,
+                                             args.spread, have_mkl)
     err = None
     if C_gpu_cols is not None:
         G = C_gpu_cols[:, : ranks * w]
         err = float(np.linalg.norm(G - Cs) / np.linalg.norm(Cs))
-    return {
-        "value": flops / secs * 1e-12,
+    del Cs
+    out = {
+        "value": flops / med * 1e-12,
         "unit": "TFLOP/s",
         "cores": ranks,
         "kind": "port",
         "sample": (f"reference stage loop (dgemm box + dtrmm triangle per band, "
                    f"{args.map} {args.bands} bands) for {ranks} ranks x {w} columns of the "
-                   f"N={n} workload ({ranks * w} of {m} columns), "
-                   f"{'MKL ' + ver.value.decode()[:40] + ' sequential' if have_mkl else 'C loops'}, "
-                   f"{secs:.2f} s"),
-        "seconds": secs,
+                   f"N={n} workload ({ranks * w} of {m} columns), {lib}, median of {CPU_RUNS} "
+                   f"runs of {med:.2f} s"),
+        "seconds": med,
+        "runs_s": [round(r, 3) for r in runs],
         "gpu_vs_cpu_rel_fro": err,
     }
+    # the reference's other two configurations: N=2048 reg P=4 (4 ranks, each
+    # its 512 columns, the whole product) and N=16384 balanced P=16 (16 ranks,
+    # a 128-column sample of each rank's 1024)
+    other = []
+    for cn, cp, cmap, cw in ((2048, 4, "reg", 512), (16384, 16, "balanced", 128)):
+        cr = min(cp, ranks)
+        cmed, cruns, _, cfl = stage_loop_median(orc, cn, cn, cp, cmap, cr, cw, args.seed,
+                                                args.spread, have_mkl)
+        other.append({"workload": f"N=M={cn} {cmap} P={cp}", "value": round(cfl / cmed * 1e-12, 4),
+                      "unit": "TFLOP/s", "cores": cr, "kind": "port",
+                      "sample": f"{cr} ranks x {cw} columns ({cr * cw} of {cn}), {lib}",
+                      "seconds": round(cmed, 4), "runs_s": [round(r, 4) for r in cruns]})
+        log(f"cpu baseline N={cn}: {cfl / cmed * 1e-12:.4f} TFLOP/s on {cr} cores")
+    out["other_configs"] = other
+    out["scalapack"] = scalapack_leg(args, C_gpu_cols, ranks)
+    # exact rows (double-double sums of the seeded inputs, generated on the fly)
+    rows = np.asarray(rows_gpu["rows"], dtype=np.int64)
+    t0 = time.time()
+    ref = np.empty((len(rows), m))
+    orc.L.orc_seeded_rows_dd(n, m, args.seed, args.spread,
+                             rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(rows),
+                             orc._p(ref), m)
+    errs = {k: float(np.linalg.norm(v - ref) / np.linalg.norm(ref))
+            for k, v in rows_gpu.items() if k != "rows"}
+    log(f"oracle rows ({len(rows)}) in {time.time() - t0:.1f}s: {errs}")
+    out["rows_checked"] = len(rows)
+    out["rows_rel_fro"] = errs
+    return out
 
 
 INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 MFMA peak
 
 
-def split_leg(args, h, uvt, dA, dB, dC, flops, plain_ms, nm):
-    """Split-precision TRMM with nm moduli on the same inputs; error against
-    the FP64 result."""
+def split_leg(args, h, uvt, dA, dB, dC, flops, plain_ms, nm, rows):
+    """Split-precision TRMM with nm moduli on the same inputs: time, the
+    call's own error statement (bound, estimate), its error against the FP64
+    result, and its sampled rows (checked against exact rows later)."""
     n, m = args.n, args.m
     ws = h.split_workspace(n, m, nm)
-    log(f"split: {nm} moduli, workspace {ws / 2**30:.1f} GiB")
+    log(f"split: {nm} moduli, workspace {ws / 2**30:.2f} GiB")
     dW, dS = h.alloc(ws), h.alloc(n * m * 8)
     for _ in range(max(1, args.warmup)):
         h.trmm_split(n, m, nm, dA, n, dB, m, dS, m, dW)
@@ -111,46 +207,45 @@ def split_leg(args, h, uvt, dA, dB, dC, flops, plain_ms, nm):
         h.record(e1)
     h.sync()
     ms = float(np.mean([h.elapsed_ms(e0, e1) for e0, e1 in evs]))
+    bound, estimate = h.split_error(n, m, nm, dW)
     S = dS.download((n, m))
     P = dC.download((n, m))
     err = float(np.linalg.norm(S - P) / np.linalg.norm(P))
+    srows = S[rows].copy()
     del S, P
     dW.free()
     dS.free()
     kb = (np.arange(n, dtype=np.float64) // 256) * 256
     int8_ops = 2.0 * m * kb.sum() * nm
     log(f"split: {ms:.2f} ms/step, {flops / (ms * 1e-3) * 1e-12:.2f} TFLOP/s-equivalent, "
-        f"rel_fro vs FP64 {err:.3e}")
+        f"rel_fro vs FP64 {err:.3e}, stated bound {bound:.3e} estimate {estimate:.3e}")
     return {"value": round(flops / (ms * 1e-3) * 1e-12, 4), "unit": "TFLOP/s (FP64-equivalent)",
-            "ms_per_step": round(ms, 4), "moduli": nm,
+            "ms_per_step": round(ms, 4), "moduli": nm, "workspace_gib": round(ws / 2**30, 3),
             "speedup_vs_plain": round(plain_ms / ms, 4), "rel_fro_vs_fp64": err,
+            "stated_error_bound": bound, "stated_error_estimate": estimate,
             "int8_ops_per_step": int8_ops,
             "int8_tops_whole_call": round(int8_ops / (ms * 1e-3) * 1e-12, 2),
-            "int8_peak_tops": INT8_PEAK_TOPS}
+            "int8_peak_tops": INT8_PEAK_TOPS}, srows
 
 
-def batch_leg(args, h, uvt):
+def batch_leg(args, h, uvt, route):
     """512 uneven TRMMs (orders log-uniform in [64, 4096], B columns uniform in
-    [16, 1024], half FP64 / half split class) in one hipGraph launch.  At
-    these orders a split-class problem runs in FP64 (faster in a batch below
-    n = 8192, and at least as accurate); int8_path_share_of_flops says how
-    much of the work the int8 path took."""
-    rng = np.random.default_rng(args.seed + 17)
-    ns = np.rint(np.exp(rng.uniform(np.log(64), np.log(4096), args.batch))).astype(int)
-    ms = rng.integers(16, 1025, args.batch)
-    order = rng.permutation(args.batch)  # which half is split: a fixed seeded permutation
-    split_set = set(order[: args.batch // 2].tolist())
+    [16, 1024], half FP64 / half split class) in one hipGraph launch.  route
+    auto: a split-class problem runs in FP64 at these orders (faster in a
+    batch below n = 8192, and at least as accurate); route int8: the split
+    half takes the int8 path wherever it has a box (n > 256).
+    int8_path_share_of_flops says how much of the work the int8 path took."""
+    probs_nm = workloads.batch_problems(args.seed, args.batch)
     table = os.path.join(ROOT, "uvatrmm_amd", "tune", "f64_tiles.json")
     table = table if os.path.exists(table) else None
     bufs, probs = [], []
-    for k in range(args.batch):
-        n, m = int(ns[k]), int(ms[k])
+    for k, (n, m, prec) in enumerate(probs_nm):
         lda, ldb = n + (n & 1), m + (m & 1)  # even leading dimensions: 16-byte loads
         dA, dB, dC = h.alloc(n * lda * 8), h.alloc(n * ldb * 8), h.alloc(n * m * 8)
         h.fill_seeded(n, m, args.seed + k, args.spread, dA, lda, dB, ldb)
         bufs += [dA, dB, dC]
-        probs.append(uvt.Problem(dA.ptr, lda, dB.ptr, ldb, dC.ptr, m, n, m,
-                                 uvt.PREC_SPLIT if k in split_set else uvt.PREC_F64, 0))
+        r = uvt.ROUTE_INT8 if (route == "int8" and prec == uvt.PREC_SPLIT) else uvt.ROUTE_AUTO
+        probs.append(uvt.Problem(dA.ptr, lda, dB.ptr, ldb, dC.ptr, m, n, m, prec, r))
     h.sync()
     t0 = time.perf_counter()
     batch = uvt.Batch(h, probs, slices=args.split_moduli or 13, table=table)
@@ -166,18 +261,27 @@ def batch_leg(args, h, uvt):
         h.record(e1)
     h.sync()
     ms_step = float(np.mean([h.elapsed_ms(e0, e1) for e0, e1 in evs]))
+    errs = [b for b, _ in batch.errors() if b >= 0]
+    ests = [e for _, e in batch.errors() if e >= 0]
     batch.close()
     for b in bufs:
         b.free()
-    f64_flops = sum(float(ms[k]) * ns[k] * ns[k] for k in range(args.batch) if k not in split_set)
-    log(f"batch: {args.batch} problems, {info['graph_nodes']} graph nodes, {ms_step:.2f} ms per "
-        f"launch, {info['flops'] / (ms_step * 1e-3) * 1e-12:.2f} TFLOP/s-equivalent")
-    return {"problems": args.batch, "value": round(info["flops"] / (ms_step * 1e-3) * 1e-12, 4),
-            "unit": "TFLOP/s (sum m n^2 / launch time)", "ms_per_launch": round(ms_step, 4),
-            "flops": info["flops"], "fp64_class_share_of_flops": round(f64_flops / info["flops"], 4),
-            "int8_path_share_of_flops": round(info["int8_flops"] / info["flops"], 4),
-            "fp64_tiles": info["fp64_tiles"], "graph_nodes": info["graph_nodes"],
-            "graph_build_s": round(build_s, 3), "tuning_table": bool(table)}
+    f64_flops = sum(float(m) * n * n for n, m, p in probs_nm if p == uvt.PREC_F64)
+    log(f"batch ({route}): {args.batch} problems, {info['graph_nodes']} graph nodes, "
+        f"{ms_step:.2f} ms per launch, {info['flops'] / (ms_step * 1e-3) * 1e-12:.2f} "
+        f"TFLOP/s-equivalent, int8 share {info['int8_flops'] / info['flops']:.3f}")
+    out = {"problems": args.batch, "route": route,
+           "value": round(info["flops"] / (ms_step * 1e-3) * 1e-12, 4),
+           "unit": "TFLOP/s (sum m n^2 / launch time)", "ms_per_launch": round(ms_step, 4),
+           "flops": info["flops"], "fp64_class_share_of_flops": round(f64_flops / info["flops"], 4),
+           "int8_path_share_of_flops": round(info["int8_flops"] / info["flops"], 4),
+           "fp64_tiles": info["fp64_tiles"], "graph_nodes": info["graph_nodes"],
+           "graph_build_s": round(build_s, 3), "tuning_table": bool(table)}
+    if errs:
+        out["int8_problems"] = len(errs)
+        out["max_stated_error_bound"] = max(errs)
+        out["max_stated_error_estimate"] = max(ests)
+    return out
 
 
 def serve_leg(args, h, uvt):
@@ -194,10 +298,7 @@ def serve_leg(args, h, uvt):
     table = table if os.path.exists(table) else None
 
     def draw(count):
-        ns = np.rint(np.exp(rng.uniform(np.log(128), np.log(16384), count))).astype(int)
-        ms = rng.integers(64, 4097, count)
-        precs = np.where(rng.random(count) < 0.5, uvt.PREC_SPLIT, uvt.PREC_F64)
-        return [(int(a), int(b), int(c)) for a, b, c in zip(ns, ms, precs)]
+        return workloads.serve_sizes(rng, count)
 
     budget = int(args.serve_budget_gb * 1e9)
     srv = uvt.Server(h, budget, max_slots=args.serve_slots, slices=args.split_moduli or 13,
@@ -243,7 +344,22 @@ def serve_leg(args, h, uvt):
     assert len(done) == total and all(d["status"] == uvt.OK for d in done)
     meas = [d for d in done if d["tag"] - 1_000_000 >= args.serve_warmup]
     lat = np.sort([d["t_done"] - d["t_submit"] for d in meas])
-    rank = lambda q: float(lat[max(1, int(np.ceil(q * len(lat)))) - 1])
+
+    def pct(v, q):  # nearest-rank percentile of a sorted array
+        return float(v[max(1, int(np.ceil(q * len(v)))) - 1])
+
+    rank = lambda q: pct(lat, q)
+    # latency by size class (m n^2): small < 1e9 <= medium < big_flops <= large
+    classes = {}
+    for name, lo, hi in (("small", 0.0, 1e9), ("medium", 1e9, args.serve_big_flops),
+                         ("large", args.serve_big_flops, float("inf"))):
+        sel = [d for d in meas
+               if lo <= float(stream[d["tag"] - 1_000_000][1]) * stream[d["tag"] - 1_000_000][0] ** 2 < hi]
+        if sel:
+            v = np.sort([d["t_done"] - d["t_submit"] for d in sel])
+            classes[name] = {"requests": len(sel), "p50_ms": round(pct(v, 0.5) * 1e3, 4),
+                             "p99_ms": round(pct(v, 0.99) * 1e3, 4)}
+    int8 = [d for d in meas if d["path"] == 1]
     span = max(d["t_done"] for d in meas) - min(d["t_submit"] for d in meas)
     fl = sum(float(stream[d["tag"] - 1_000_000][1]) * stream[d["tag"] - 1_000_000][0] ** 2
              for d in meas)
@@ -258,7 +374,11 @@ def serve_leg(args, h, uvt):
            "large_request_flops": args.serve_big_flops, "max_large_in_flight": args.serve_max_big,
            "peak_arena_gb": round(st["peak_bytes"] / 1e9, 3),
            "peak_inflight": st["peak_inflight"],
-           "split_share": round(float(np.mean([p for _, _, p in stream])), 3),
+           "split_class_share": round(float(np.mean([p for _, _, p in stream])), 3),
+           "int8_path_share": round(len(int8) / len(meas), 4),
+           "by_size_class": classes,
+           "int8_max_stated_error_bound": max((d["err_bound"] for d in int8), default=None),
+           "int8_max_stated_error_estimate": max((d["err_estimate"] for d in int8), default=None),
            "latency": "submit -> result ready in HBM (inputs generated on device per request)"}
     log(f"serve: p50 {out['p50_ms']} ms p99 {out['p99_ms']} ms, {out['throughput_tflops']} "
         f"TFLOP/s, {out['requests_per_s']} req/s, peak {st['peak_inflight']} in flight")
@@ -295,6 +415,19 @@ def configs_leg(args, h, uvt):
     return out
 
 
+def sample_rows(n, bands, mapping):
+    """Rows of C checked against exact rows: the ends of the first, middle and
+    last 256-row tiles, the 8192-row boundaries, every band start, and a few
+    seeded others."""
+    import uvatrmm_amd as uvt
+
+    rows = {0, 255, 256, n // 4 - 1, n // 4, n // 2 - 1, n // 2, n - 257, n - 256, n - 1}
+    rows |= {b for b, _ in uvt.plan_bands(n, bands, mapping)}
+    rng = np.random.default_rng(n + 5)
+    rows |= set(int(x) for x in rng.integers(0, n, 8))
+    return sorted(r for r in rows if 0 <= r < n)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -308,7 +441,7 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--spread", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-cols-per-rank", type=int, default=256)
+    ap.add_argument("--cpu-cols-per-rank", type=int, default=128)
     ap.add_argument("--batch", type=int, default=512, help="problems of the batch leg (0 = skip)")
     ap.add_argument("--split-moduli", type=int, default=12,
                     help="moduli of the split-precision leg (0 = skip it); 12 keeps the "
@@ -364,15 +497,25 @@ def main():
     log(f"steps ms: min {min(step_ms):.2f} mean {mean_ms:.2f} max {max(step_ms):.2f}; "
         f"wall/step {wall / args.steps * 1e3:.2f} ms; {achieved:.2f} TFLOP/s")
     norm = h.inf_norm(n, m, dC, m)
+    rows = sample_rows(n, args.bands, args.map)
+    rows_gpu = {"rows": rows}
+    full = dC.download((n, m)) if m * n * 8 <= (1 << 34) else None
+    if full is not None:
+        rows_gpu["fp64"] = full[rows].copy()
+        del full
 
     split, split_alt = None, []
     if args.split_moduli > 0:
-        split = split_leg(args, h, uvt, dA, dB, dC, flops, mean_ms, args.split_moduli)
+        split, rows_gpu[f"split{args.split_moduli}"] = split_leg(args, h, uvt, dA, dB, dC, flops,
+                                                                  mean_ms, args.split_moduli, rows)
         for nm in args.split_moduli_alt:
             if nm != args.split_moduli:
-                split_alt.append(split_leg(args, h, uvt, dA, dB, dC, flops, mean_ms, nm))
+                leg, rows_gpu[f"split{nm}"] = split_leg(args, h, uvt, dA, dB, dC, flops, mean_ms,
+                                                        nm, rows)
+                split_alt.append(leg)
 
-    batch = batch_leg(args, h, uvt) if args.batch > 0 else None
+    batch = batch_leg(args, h, uvt, "auto") if args.batch > 0 else None
+    batch_forced = batch_leg(args, h, uvt, "int8") if args.batch > 0 else None
 
     other = configs_leg(args, h, uvt) if not args.no_configs else None
 
@@ -381,17 +524,22 @@ def main():
     cpu = None
     if not args.no_cpu:
         cols = min(m, 16 * args.cpu_cols_per_rank)
-        Cg = np.empty((n, cols))
+        Cg = None
         full = dC.download((n, m)) if m * n * 8 <= (1 << 34) else None
         if full is not None:
             Cg = np.ascontiguousarray(full[:, :cols])
             del full
-        else:
-            Cg = None
         try:
-            cpu = cpu_baseline(args, Cg, cols)
+            cpu = cpu_baseline(args, Cg, rows_gpu)
             log(f"cpu baseline {cpu['value']:.4f} TFLOP/s on {cpu['cores']} cores; "
                 f"gpu vs cpu rel_fro {cpu['gpu_vs_cpu_rel_fro']}")
+            if split is not None:
+                split["rel_fro_vs_exact_rows"] = cpu["rows_rel_fro"].get(f"split{args.split_moduli}")
+            for leg in split_alt:
+                leg["rel_fro_vs_exact_rows"] = cpu["rows_rel_fro"].get(f"split{leg['moduli']}")
+            if other is not None:
+                for g, c in zip(other, cpu["other_configs"]):
+                    g["cpu_baseline"] = c
         except Exception as e:  # the baseline is reported, never the measured path
             log("cpu baseline failed:", repr(e))
 
@@ -422,6 +570,7 @@ def main():
         "split": split,
         "split_other_moduli": split_alt,
         "batch": batch,
+        "batch_forced_split": batch_forced,
         "other_configs": other,
         "serve": serve,
         "norm": norm,

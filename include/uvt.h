@@ -38,6 +38,14 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+/* opaque handles, declared before the export region: their members are the
+ * library's own */
+struct uvt_plan_s;
+struct uvt_handle_s;
+struct uvt_batch_s;
+struct uvt_server_s;
+/* the library builds with hidden symbols; these declarations are its exports */
+#pragma GCC visibility push(default)
 
 typedef enum {
     UVT_OK = 0,
@@ -121,7 +129,12 @@ uvt_status uvt_destroy(uvt_handle h);
 int32_t uvt_device_cus(uvt_handle h);
 
 /* FP64 C = tril(A) * B following the plan (out of place; A and B are not
- * written; C is fully overwritten on rows [0, n) x columns [0, m)). */
+ * written; C is fully overwritten on rows [0, n) x columns [0, m)).  Entries
+ * of A above the diagonal are never read.  The kernels multiply whole
+ * 16-deep k panels of B, so a NaN or infinity in B[k][j] also reaches the
+ * rows above k in k's 128-row tile through A's structural zeros (0 * Inf =
+ * NaN), where a BLAS dtrmm would leave them finite; finite inputs are not
+ * affected. */
 uvt_status uvt_trmm_f64(uvt_handle h, uvt_plan plan, const double *dA, int64_t lda,
                         const double *dB, int64_t ldb, double *dC, int64_t ldc, void *stream);
 
@@ -135,6 +148,20 @@ uvt_status uvt_trmm_split_workspace(int64_t n, int64_t m, int32_t slices, size_t
 uvt_status uvt_trmm_split(uvt_handle h, int64_t n, int64_t m, int32_t slices, const double *dA,
                           int64_t lda, const double *dB, int64_t ldb, double *dC, int64_t ldc,
                           void *dwork, size_t work_bytes, void *stream);
+/* The error statement of the last uvt_trmm_split run on this workspace with
+ * these n, m, slices (synchronous on `stream`), both relative to ||C||_F and
+ * computed on the device from the scales and norms of that call:
+ *   bound:    rigorous, >= ||C - tril(A) B||_F / ||tril(A) B||_F (from the
+ *             worst-case integer roundings, the FP64 diagonal blocks and the
+ *             final roundings; inf if it exceeds ||C||_F);
+ *   estimate: the expected value of that error when the roundings to integers
+ *             are independent and uniform (random inputs measure ~0.9x of it;
+ *             the bound is typically 30-150x larger).
+ * Both are NaN when A or B holds a NaN or infinity in the part the integer
+ * products read; those entries of C are then NaN. */
+uvt_status uvt_trmm_split_error(uvt_handle h, int64_t n, int64_t m, int32_t slices,
+                                const void *dwork, size_t work_bytes, double *bound,
+                                double *estimate, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* BLAS forms of the two per-band calls, device-resident and row-major */
@@ -147,7 +174,9 @@ typedef enum { UVT_NON_UNIT = 0, UVT_UNIT = 1 } uvt_diag;
 /* C = alpha op(A) op(B) + beta C; C (m x n) is not read when beta == 0, A and
  * B are not read when k == 0 or alpha == 0.  Replaces the cblas_dgemm call of
  * UVaTRMM/src/mm_triangular.c:347-360 (RowMajor, NoTrans, NoTrans, alpha 1,
- * beta 0) and covers the other transposes.  Uses the handle's workspace. */
+ * beta 0) and covers the other transposes.  Uses the handle's workspace:
+ * calls on one handle that use it (uvt_dgemm, uvt_dtrmm, uvt_inf_norm) must
+ * be on one stream or serialized by the caller. */
 uvt_status uvt_dgemm(uvt_handle h, uvt_trans ta, uvt_trans tb, int64_t m, int64_t n, int64_t k,
                      double alpha, const double *dA, int64_t lda, const double *dB, int64_t ldb,
                      double beta, double *dC, int64_t ldc, void *stream);
@@ -156,7 +185,7 @@ uvt_status uvt_dgemm(uvt_handle h, uvt_trans ta, uvt_trans tb, int64_t m, int64_
  * diagonal is taken as 1).  Replaces the cblas_dtrmm call of
  * mm_triangular.c:374-387 (RowMajor, Left, Lower, NoTrans, NonUnit, alpha 1)
  * and covers every other side / uplo / trans / diag.  Uses the handle's
- * workspace. */
+ * workspace (see uvt_dgemm). */
 uvt_status uvt_dtrmm(uvt_handle h, uvt_side side, uvt_uplo uplo, uvt_trans ta, uvt_diag diag,
                      int64_t m, int64_t n, double alpha, const double *dA, int64_t lda,
                      double *dB, int64_t ldb, void *stream);
@@ -190,8 +219,14 @@ typedef struct {
     int64_t ldc;
     int64_t n, m;
     int32_t prec;    /* UVT_PREC_F64 or UVT_PREC_SPLIT */
-    int32_t reserved;
+    int32_t route;   /* UVT_ROUTE_*: where a split-class problem's box part runs */
 } uvt_problem;
+
+/* auto: the int8 path where it is measured faster in a batch (n >= 8192 and
+ * m >= 256), FP64 elsewhere; int8: the int8 path wherever it supports the
+ * size (n > 256, n <= 131072, enough moduli for the depth); fp64: FP64.  FP64
+ * problems take auto or fp64. */
+enum { UVT_ROUTE_AUTO = 0, UVT_ROUTE_INT8 = 1, UVT_ROUTE_FP64 = 2 };
 
 typedef struct uvt_batch_s *uvt_batch;
 
@@ -209,11 +244,16 @@ uvt_status uvt_batch_run(uvt_batch b, void *stream);
  * device memory, so the captured graph picks them up); synchronous. */
 uvt_status uvt_batch_update(uvt_batch b, const uvt_problem *probs, int32_t count);
 /* Algorithmic flops (sum of m n^2), the part of them whose box runs on the
- * int8 path (split-class problems large enough for it to be faster than FP64
- * in a batch: n >= 8192 and m >= 256; the others run in FP64), FP64 tiles
- * queued, graph nodes.  Any pointer may be NULL. */
+ * int8 path (see uvt_problem.route), FP64 tiles queued, graph nodes.  Any
+ * pointer may be NULL. */
 uvt_status uvt_batch_info(uvt_batch b, double *flops, double *int8_flops, int64_t *fp64_tiles,
                           int32_t *graph_nodes);
+/* The error statement of every problem after the last run (synchronous):
+ * for problems whose box ran on the int8 path, the rigorous bound and the
+ * expected value of the relative Frobenius error (see uvt_trmm_split_error);
+ * -1 for problems that ran wholly in FP64.  Arrays of count entries (either
+ * may be NULL). */
+uvt_status uvt_batch_errors(uvt_batch b, double *bound, double *estimate, int32_t count);
 uvt_status uvt_batch_destroy(uvt_batch b);
 
 /* FP64 tile shapes selectable per problem. */
@@ -275,8 +315,9 @@ enum {
 typedef struct {
     uint64_t ticket, tag;
     uvt_status status;
-    int32_t reserved;
+    int32_t path;                     /* 1: the box ran on the int8 path, 0: FP64 */
     double t_submit, t_admit, t_done; /* seconds since the server started */
+    double err_bound, err_estimate;   /* int8 path: as uvt_trmm_split_error; FP64: -1 */
 } uvt_completion;
 
 typedef struct {
@@ -342,6 +383,7 @@ uvt_status uvt_event_record(uvt_handle h, void *event, void *stream);
 /* Milliseconds between two recorded events (synchronises on `end`). */
 uvt_status uvt_event_elapsed(uvt_handle h, void *start, void *end, float *ms);
 
+#pragma GCC visibility pop
 #ifdef __cplusplus
 }
 #endif

@@ -24,6 +24,8 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+/* the library builds with hidden symbols; these declarations are its exports */
+#pragma GCC visibility push(default)
 
 /* Callers that already include a cblas.h define UVT_CBLAS_NO_ENUMS. */
 #ifndef UVT_CBLAS_NO_ENUMS
@@ -44,6 +46,7 @@ void cblas_dtrmm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, CBLAS_TR
                  CBLAS_DIAG diag, int m, int n, double alpha, const double *a, int lda, double *b,
                  int ldb);
 
+#pragma GCC visibility pop
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,23 @@ def test_library_exports_nothing_undeclared(uvt, header):
             exported.add(parts[2])
     extra = sorted(exported - set(declared(header)))
     assert not extra, extra
+    # C++ symbols of the library's own (kernels, their launch stubs, helper
+    # functions and types) are hidden too; only instantiations of the
+    # standard library's templates may remain
+    cxx = subprocess.run(["nm", "-DC", "--defined-only", path], capture_output=True, text=True,
+                         timeout=60, check=True).stdout
+    own = []
+    for line in cxx.splitlines():
+        parts = line.split(None, 2)
+        if len(parts) < 3 or parts[1] not in "TtWVDBR":
+            continue
+        name = parts[2]
+        if name.startswith(("std::", "vtable for std::", "typeinfo for std::",
+                            "typeinfo name for std::", "void std::", "__gnu_cxx::", "__hip_cuid_")):
+            continue
+        if "(" in name or "::" in name or name.startswith(("k_", "__device_stub")):
+            own.append(name)
+    assert not own, own
 
 
 def test_version_and_status_strings(uvt):
@@ -73,6 +90,20 @@ def test_cli_usage_errors_exit_3(uvt):
 
 def test_cli_mapping_abort_codes(uvt):
     assert _cli("7", "balanced", "boxes", "--bands", "7").returncode == 11
+
+
+def test_split_routing_falls_back_to_fp64(uvt):
+    """A split-class request too deep for its moduli (n = 65536 with 2 moduli
+    leaves fewer than 6 bits per operand) runs in FP64 instead of failing:
+    the server charges it no split workspace."""
+    def nbytes(n, m, prec, slices):
+        import ctypes
+        cfg = uvt.ServeConfig(1 << 40, 1, slices, None, 0.0, 0, 0)
+        b = ctypes.c_size_t()
+        uvt.check(uvt.lib().uvt_serve_request_bytes(ctypes.byref(cfg), n, m, prec, ctypes.byref(b)))
+        return b.value
+    assert nbytes(65536, 2048, uvt.PREC_SPLIT, 2) == nbytes(65536, 2048, uvt.PREC_F64, 2)
+    assert nbytes(65536, 2048, uvt.PREC_SPLIT, 13) > nbytes(65536, 2048, uvt.PREC_F64, 13)
 
 
 def test_serve_request_bytes(uvt):

@@ -115,3 +115,114 @@ def test_batch_update_new_pointers(uvt, gpu, oracle):
     for dB2, dC2, _ in keep:
         dB2.free()
         dC2.free()
+
+
+def test_batch_update_one_int8_problem(uvt, gpu, oracle):
+    """A batch whose only int8-path problem is (8200, 260) beside a small FP64
+    one: after uvt_batch_update with new B and C buffers, the captured graph's
+    split stages must read the new B and write the new C (the int8 problem's
+    stages read it from the device descriptor table), checked against the
+    oracle; the old C must stay untouched by the second run."""
+    sizes = [(300, 40), (8200, 260)]
+    precs = [uvt.PREC_F64, uvt.PREC_SPLIT]
+    host, bufs, probs = [], [], []
+    for k, ((n, m), prec) in enumerate(zip(sizes, precs)):
+        A = oracle.seeded_A(n, 300 + k, 1)
+        B = oracle.seeded_B(n, m, 300 + k, 1)
+        dA, dB, dC = gpu.array(A), gpu.array(B), gpu.alloc(n * m * 8)
+        host.append((A, B))
+        bufs.append((dA, dB, dC))
+        probs.append(uvt.Problem(dA.ptr, n, dB.ptr, m, dC.ptr, m, n, m, prec, 0))
+    batch = uvt.Batch(gpu, probs, slices=13)
+    assert batch.info()["int8_flops"] == 260.0 * 8200 * 8200
+    batch.run()
+    gpu.sync()
+    first = [bufs[k][2].download((n, m)) for k, (n, m) in enumerate(sizes)]
+    new, keep = [], []
+    for k, ((n, m), prec) in enumerate(zip(sizes, precs)):
+        B2 = oracle.seeded_B(n, m, 900 + k, 1)
+        dB2, dC2 = gpu.array(B2), gpu.alloc(n * m * 8)
+        keep.append((dB2, dC2, B2))
+        new.append(uvt.Problem(bufs[k][0].ptr, n, dB2.ptr, m, dC2.ptr, m, n, m, prec, 0))
+    batch.update(new)
+    batch.run()
+    gpu.sync()
+    errs = batch.errors()
+    for k, (n, m) in enumerate(sizes):
+        ref = oracle.trmm(host[k][0], keep[k][2])
+        err = rel_fro(keep[k][1].download((n, m)), ref)
+        say("updated problem", k, (n, m), "err", err, "stated", errs[k])
+        assert err <= (1e-13 if precs[k] == uvt.PREC_F64 else 1e-12)
+        np.testing.assert_array_equal(bufs[k][2].download((n, m)), first[k])
+    assert errs[0] == (-1.0, -1.0)
+    assert errs[1][0] >= rel_fro(keep[1][1].download(sizes[1]), oracle.trmm(host[1][0], keep[1][2]))
+    batch.close()
+    for t in bufs:
+        for b in t:
+            b.free()
+    for dB2, dC2, _ in keep:
+        dB2.free()
+        dC2.free()
+
+
+def test_bench_batch_both_routes(uvt, gpu, oracle):
+    """The benchmark's 512-problem batch (seed 3), with the shipped tuning
+    table, routed by measured speed (every problem in FP64 at these orders)
+    and with its split half forced onto the int8 path: FP64-class results
+    are the same bits in both; each forced split result is within 1e-12 of
+    the routed FP64 one; a sample of 24 problems (the 12 largest, 12 seeded
+    others) is checked against the oracle, and every int8-path problem's
+    stated bound covers its error against the oracle on that sample."""
+    from uvatrmm_amd import workloads
+    import os
+
+    probs_nm = workloads.batch_problems(3, 512)
+    table = os.path.join(os.path.dirname(uvt.__file__), "tune", "f64_tiles.json")
+    bufs, results, errs, infos = [], {}, {}, {}
+    for k, (n, m, prec) in enumerate(probs_nm):
+        lda, ldb = n + (n & 1), m + (m & 1)
+        dA, dB = gpu.alloc(n * lda * 8), gpu.alloc(n * ldb * 8)
+        gpu.fill_seeded(n, m, 3 + k, 1, dA, lda, dB, ldb)
+        bufs.append((dA, dB, lda, ldb))
+    for route in ("auto", "int8"):
+        outs, probs = [], []
+        for k, (n, m, prec) in enumerate(probs_nm):
+            dA, dB, lda, ldb = bufs[k]
+            dC = gpu.alloc(n * m * 8)
+            outs.append(dC)
+            r = uvt.ROUTE_INT8 if (route == "int8" and prec == uvt.PREC_SPLIT) else uvt.ROUTE_AUTO
+            probs.append(uvt.Problem(dA.ptr, lda, dB.ptr, ldb, dC.ptr, m, n, m, prec, r))
+        batch = uvt.Batch(gpu, probs, slices=12, table=table)
+        infos[route] = batch.info()
+        batch.run()
+        gpu.sync()
+        errs[route] = batch.errors()
+        batch.close()
+        results[route] = [outs[k].download((n, m)) for k, (n, m, _) in enumerate(probs_nm)]
+        for b in outs:
+            b.free()
+    say("auto", infos["auto"], "int8", infos["int8"])
+    assert infos["auto"]["int8_flops"] == 0.0
+    assert infos["int8"]["int8_flops"] > 0.4 * infos["int8"]["flops"]
+    for k, (n, m, prec) in enumerate(probs_nm):
+        if prec == uvt.PREC_F64:
+            np.testing.assert_array_equal(results["int8"][k], results["auto"][k], err_msg=str(k))
+        else:
+            assert rel_fro(results["int8"][k], results["auto"][k]) <= 1e-12, (k, n, m)
+            assert (errs["int8"][k][0] >= 0) == (n > 256)
+    order = sorted(range(512), key=lambda k: -probs_nm[k][1] * probs_nm[k][0] ** 2)
+    rng = np.random.default_rng(5)
+    sample = sorted(set(order[:12]) | set(int(x) for x in rng.choice(512, 12, replace=False)))
+    for k in sample:
+        n, m, prec = probs_nm[k]
+        ref = oracle.trmm(oracle.seeded_A(n, 3 + k, 1), oracle.seeded_B(n, m, 3 + k, 1))
+        for route in ("auto", "int8"):
+            err = rel_fro(results[route][k], ref)
+            tol = 1e-13 if (prec == uvt.PREC_F64 or route == "auto") else 1e-12
+            assert err <= tol, (k, n, m, prec, route, err)
+            bound = errs[route][k][0]
+            if bound >= 0:
+                assert err <= bound, (k, n, m, err, bound)
+    for dA, dB, _, _ in bufs:
+        dA.free()
+        dB.free()

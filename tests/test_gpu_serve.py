@@ -49,17 +49,17 @@ def serve(uvt, gpu, reqs, budget, slots):
     assert sorted(d["tag"] for d in done) == list(range(len(reqs)))
     for d in done:
         assert d["t_submit"] <= d["t_admit"] <= d["t_done"]
-    return outs, st
+    return outs, st, {d["tag"]: d for d in done}
 
 
 def test_serve_alone_equals_mix(uvt, gpu, oracle):
     # the last two are split-class requests large enough for the int8 path
     reqs = mixed_requests(40, seed=11) + [(4096, 1024, 1, 6001), (6144, 640, 1, 6002)]
     say("requests", len(reqs), "alone")
-    alone, st1 = serve(uvt, gpu, reqs, 8 * GiB, 1)
+    alone, st1, _ = serve(uvt, gpu, reqs, 8 * GiB, 1)
     assert st1["peak_inflight"] == 1
     say("mix, 8 slots")
-    mix, st8 = serve(uvt, gpu, reqs, 8 * GiB, 8)
+    mix, st8, comp = serve(uvt, gpu, reqs, 8 * GiB, 8)
     say("stats", st8)
     assert st8["peak_inflight"] > 1
     assert st8["completed"] == st8["submitted"] == len(reqs)
@@ -69,6 +69,26 @@ def test_serve_alone_equals_mix(uvt, gpu, oracle):
         ref = oracle.trmm(oracle.seeded_A(n, seed, 1), oracle.seeded_B(n, m, seed, 1))
         err = rel_fro(mix[k], ref)
         assert err <= (1e-13 if prec == 0 else 1e-12), (k, n, m, prec, err)
+    # the two large split-class requests ran on the int8 path and state their
+    # error: the bound covers the error against the oracle; every other
+    # request ran in FP64 and states none
+    for k, (n, m, prec, seed) in enumerate(reqs):
+        int8 = k >= len(reqs) - 2
+        assert comp[k]["path"] == (1 if int8 else 0), (k, n, m, prec)
+        if not int8:
+            assert comp[k]["err_bound"] == -1.0
+            continue
+        # exact rows (double-double sums) of a sample: at 13 moduli the split
+        # error is below the rounding error of a plain FP64 product
+        rng = np.random.default_rng(k)
+        rows = np.unique(np.concatenate([np.arange(0, n, 256), rng.integers(0, n, 160)]))
+        A, B = oracle.seeded_A(n, seed, 1), oracle.seeded_B(n, m, seed, 1)
+        err = rel_fro(mix[k][rows], oracle.trmm_rows_dd(A, B, rows))
+        say("int8 request", (n, m), "err", err, "bound", comp[k]["err_bound"],
+            "estimate", comp[k]["err_estimate"])
+        assert err <= 1e-12
+        assert err <= comp[k]["err_bound"]
+        assert 0.25 <= err / comp[k]["err_estimate"] <= 4.0
 
 
 def test_serve_budget_admission(uvt, gpu):
@@ -76,9 +96,9 @@ def test_serve_budget_admission(uvt, gpu):
     need = max(uvt.serve_request_bytes(n, m, p) for n, m, p, _ in reqs)
     budget = int(need * 1.5)
     say("budget", budget, "largest", need)
-    outs, st = serve(uvt, gpu, reqs, budget, 16)
+    outs, st, _ = serve(uvt, gpu, reqs, budget, 16)
     assert st["peak_bytes"] <= budget
-    alone, _ = serve(uvt, gpu, reqs[:6], 8 * GiB, 1)
+    alone, _, _ = serve(uvt, gpu, reqs[:6], 8 * GiB, 1)
     for k in range(6):
         assert np.array_equal(alone[k], outs[k])
 
@@ -180,7 +200,7 @@ def test_serve_large_request_cap(uvt, gpu, oracle):
         assert b["t_admit"] >= a["t_done"], times
     assert done[2]["t_submit"] < done[0]["t_done"], times  # the scenario: 0 still running
     assert done[2]["t_admit"] < done[1]["t_admit"], times  # the small one passed the held-back large one
-    alone, _ = serve(uvt, gpu, reqs, 8 * GiB, 1)
+    alone, _, _ = serve(uvt, gpu, reqs, 8 * GiB, 1)
     for k, (n, m, prec, seed) in enumerate(reqs):
         assert np.array_equal(alone[k], outs[k]), (k, n, m)
     for k in (2, 6):

@@ -39,7 +39,7 @@ class Tile(_c.Structure):
 
 class Problem(_c.Structure):
     _fields_ = [("A", vp), ("lda", i64), ("B", vp), ("ldb", i64), ("C", vp), ("ldc", i64),
-                ("n", i64), ("m", i64), ("prec", i32), ("reserved", i32)]
+                ("n", i64), ("m", i64), ("prec", i32), ("route", i32)]
 
 
 class ServeConfig(_c.Structure):
@@ -53,8 +53,9 @@ class Request(_c.Structure):
 
 
 class Completion(_c.Structure):
-    _fields_ = [("ticket", u64), ("tag", u64), ("status", i32), ("reserved", i32),
-                ("t_submit", f64), ("t_admit", f64), ("t_done", f64)]
+    _fields_ = [("ticket", u64), ("tag", u64), ("status", i32), ("path", i32),
+                ("t_submit", f64), ("t_admit", f64), ("t_done", f64), ("err_bound", f64),
+                ("err_estimate", f64)]
 
 
 class ServeStats(_c.Structure):
@@ -64,6 +65,7 @@ class ServeStats(_c.Structure):
 
 
 PREC_F64, PREC_SPLIT = 0, 1
+ROUTE_AUTO, ROUTE_INT8, ROUTE_FP64 = 0, 1, 2
 REQ_AB_DEVICE, REQ_C_DEVICE = 1, 2
 TILE_CFGS = {0: "128x128", 1: "64x64", 2: "128x32"}
 
@@ -98,11 +100,13 @@ _SIGS = {
     "uvt_trmm_f64": (i32, [vp, vp, vp, i64, vp, i64, vp, i64, vp]),
     "uvt_trmm_split_workspace": (i32, [i64, i64, i32, _c.POINTER(sz)]),
     "uvt_trmm_split": (i32, [vp, i64, i64, i32, vp, i64, vp, i64, vp, i64, vp, sz, vp]),
+    "uvt_trmm_split_error": (i32, [vp, i64, i64, i32, vp, sz, _c.POINTER(f64), _c.POINTER(f64), vp]),
     "uvt_batch_create": (i32, [vp, _c.c_void_p, i32, i32, _c.c_char_p, _c.POINTER(vp)]),
     "uvt_batch_run": (i32, [vp, vp]),
     "uvt_batch_update": (i32, [vp, _c.c_void_p, i32]),
     "uvt_batch_info": (i32, [vp, _c.POINTER(f64), _c.POINTER(f64), _c.POINTER(i64),
                              _c.POINTER(i32)]),
+    "uvt_batch_errors": (i32, [vp, _c.POINTER(f64), _c.POINTER(f64), i32]),
     "uvt_batch_destroy": (i32, [vp]),
     "uvt_autotune_f64": (i32, [vp, _c.c_char_p]),
     "uvt_tune_choice": (i32, [_c.c_char_p, i64, i64, _c.POINTER(i32)]),
@@ -256,6 +260,17 @@ class DeviceBuffer:
         check(lib().uvt_memcpy(self.h.ptr, out.ctypes.data, self.ptr, out.nbytes, 1), "d2h")
         return out
 
+    def download_rows(self, rows, ncols, ld=None, dtype=np.float64):
+        """Selected rows of a row-major matrix with leading dimension ld."""
+        ld = ld or ncols
+        isz = np.dtype(dtype).itemsize
+        out = np.empty((len(rows), ncols), dtype=dtype)
+        for k, r in enumerate(rows):
+            assert (int(r) * ld + ncols) * isz <= self.nbytes
+            check(lib().uvt_memcpy(self.h.ptr, out[k].ctypes.data, self.ptr + int(r) * ld * isz,
+                                   ncols * isz, 1), "d2h rows")
+        return out
+
     def zero(self):
         check(lib().uvt_memset(self.h.ptr, self.ptr, 0, self.nbytes))
 
@@ -301,6 +316,14 @@ class Handle:
     def trmm_split(self, n, m, slices, A, lda, B, ldb, C, ldc, work):
         check(lib().uvt_trmm_split(self.ptr, n, m, slices, A.ptr, lda, B.ptr, ldb, C.ptr, ldc,
                                    work.ptr, work.nbytes, self.stream), "trmm_split")
+
+    def split_error(self, n, m, slices, work):
+        """(bound, estimate) of the relative Frobenius error of the last
+        trmm_split on `work` (see uvt_trmm_split_error)."""
+        b, e = f64(), f64()
+        check(lib().uvt_trmm_split_error(self.ptr, n, m, slices, work.ptr, work.nbytes, _c.byref(b),
+                                         _c.byref(e), self.stream), "split_error")
+        return b.value, e.value
 
     def inf_norm(self, n, m, C, ldc):
         out = f64()
@@ -375,6 +398,12 @@ class Batch:
         return {"flops": fl.value, "int8_flops": i8.value, "fp64_tiles": tiles.value,
                 "graph_nodes": nodes.value}
 
+    def errors(self):
+        """Per problem (bound, estimate) of the last run; -1 for FP64 ones."""
+        b, e = (f64 * self.count)(), (f64 * self.count)()
+        check(lib().uvt_batch_errors(self.ptr, b, e, self.count), "batch_errors")
+        return list(zip(b, e))
+
     def close(self):
         if self.ptr:
             lib().uvt_batch_destroy(self.ptr)
@@ -447,8 +476,9 @@ class Server:
         res = []
         for c in out[:k.value]:
             self._host.pop(c.ticket, None)
-            res.append({"ticket": c.ticket, "tag": c.tag, "status": c.status,
-                        "t_submit": c.t_submit, "t_admit": c.t_admit, "t_done": c.t_done})
+            res.append({"ticket": c.ticket, "tag": c.tag, "status": c.status, "path": c.path,
+                        "t_submit": c.t_submit, "t_admit": c.t_admit, "t_done": c.t_done,
+                        "err_bound": c.err_bound, "err_estimate": c.err_estimate})
         return res
 
     def drain(self):

@@ -15,6 +15,8 @@ using namespace uvt;
 // Kernels
 // ---------------------------------------------------------------------
 
+namespace {  // kernels: internal to this file
+
 template <int TM, int TN, bool VEC>
 __global__ __launch_bounds__((64 * TileWaves<TM, TN>::M * TileWaves<TM, TN>::N),
                              (TileWaves<TM, TN>::WAVES_PER_EU))
@@ -135,6 +137,8 @@ __global__ void k_fill_seeded(int64_t rows, int64_t cols, uint64_t key, int stre
         X[i * ldx + j] = v;
     }
 }
+
+}  // namespace
 
 // ---------------------------------------------------------------------
 // Handle

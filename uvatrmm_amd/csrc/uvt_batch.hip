@@ -12,9 +12,13 @@
 //   -> join -> the split reconstruction, which adds the integer box part to
 //              the diagonal blocks the FP64 queue stored.
 // The FP64 queue holds every tile of the FP64 problems and the diagonal-block
-// tiles of the split problems.  Split-class problems for which the int8 path
-// is not faster in a batch (uvt_split_faster_in_batch) run wholly in FP64,
-// which meets their tolerance.
+// tiles of the split problems.  Split-class problems run wholly in FP64,
+// which meets their tolerance, unless their route asks for the int8 path or
+// (route auto) the int8 path is measured faster in a batch
+// (uvt_split_faster_in_batch); sizes the int8 path does not support always
+// run in FP64.  The split stages read every problem from a device
+// descriptor table, so uvt_batch_update reaches them under the captured
+// graph.
 // Every tile and every split stage works on one problem with a fixed
 // k-order, so a problem's result is the same alone or in any batch.
 #include <hip/hip_runtime.h>
@@ -241,7 +245,8 @@ struct uvt_batch_s {
     size_t split_work_bytes = 0;
     void* d_split_desc = nullptr;
     int* d_split_lut = nullptr;
-    int64_t split_totals[8] = {0}, split_lut_off[8] = {0};
+    int64_t split_totals[UVT_SPLIT_STAGES] = {0}, split_lut_off[UVT_SPLIT_STAGES] = {0};
+    std::vector<double*> split_err;  // per int8-path problem: its error statement (device)
     // graph
     hipStream_t cap = nullptr, side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
@@ -276,8 +281,22 @@ static uvt_status check(const uvt_problem& p)
     if (!p.A || !p.B || !p.C || p.n <= 0 || p.m <= 0 || p.lda < p.n || p.ldb < p.m || p.ldc < p.m)
         return UVT_ERR_ARG;
     if (p.prec != UVT_PREC_F64 && p.prec != UVT_PREC_SPLIT) return UVT_ERR_ARG;
+    if (p.route != UVT_ROUTE_AUTO && p.route != UVT_ROUTE_INT8 && p.route != UVT_ROUTE_FP64)
+        return UVT_ERR_ARG;
+    if (p.prec == UVT_PREC_F64 && p.route == UVT_ROUTE_INT8) return UVT_ERR_ARG;
     if (p.n > 0x3FFFFFFF || p.m > 0x3FFFFFFF) return UVT_ERR_SIZE;
     return UVT_OK;
+}
+
+// Whether a problem's box part runs on the int8 path: split class, a box to
+// run (n > 256), a size the int8 path supports with these moduli (else FP64,
+// which meets the split tolerance), and either asked for or measured faster
+// in a batch.
+static bool routes_to_int8(const uvt_problem& p, int slices)
+{
+    if (p.prec != UVT_PREC_SPLIT || p.route == UVT_ROUTE_FP64 || p.n <= 256) return false;
+    if (!uvt_split_supported(p.n, slices)) return false;
+    return p.route == UVT_ROUTE_INT8 || uvt_split_faster_in_batch(p.n, p.m);
 }
 
 static BatchProb f64_desc(const uvt_problem& p)
@@ -302,7 +321,7 @@ static uvt_status upload_split(uvt_batch b)
     std::vector<char> desc;
     std::vector<int> lut;
     uvt_status st = uvt_split_batch_build(b->sp.data(), (int)b->sp.size(), b->slices, b->d_split_work,
-                                          desc, lut, b->split_totals, b->split_lut_off);
+                                          desc, lut, b->split_totals, b->split_lut_off, &b->split_err);
     if (st != UVT_OK) return st;
     UVT_HIP(hipMemcpy(b->d_split_desc, desc.data(), desc.size(), hipMemcpyHostToDevice));
     if (!lut.empty())
@@ -314,6 +333,7 @@ extern "C" uvt_status uvt_batch_create(uvt_handle h, const uvt_problem* probs, i
                                        int32_t slices, const char* tune_table, uvt_batch* out)
 {
     if (!h || !probs || count <= 0 || !out) return UVT_ERR_ARG;
+    if (slices < 2 || slices > 14) return UVT_ERR_ARG;
     for (int i = 0; i < count; ++i) {
         uvt_status st = check(probs[i]);
         if (st != UVT_OK) return st;
@@ -338,7 +358,7 @@ extern "C" uvt_status uvt_batch_create(uvt_handle h, const uvt_problem* probs, i
         b->f64_index.push_back(slot);
         b->f64_probs.push_back(f64_desc(p));
         const int cfg = table.empty() ? uvt_default_cfg(p.n, p.m) : uvt_tune_pick(table, p.n, p.m);
-        const bool use_int8 = p.prec == UVT_PREC_SPLIT && uvt_split_faster_in_batch(p.n, p.m);
+        const bool use_int8 = routes_to_int8(p, slices);
         trapezoid_tiles(slot, p.n, p.m, cfg, b->tiles[cfg * 2 + b->f64_probs.back().vec], use_int8);
         b->int8.push_back(use_int8);
         if (use_int8) b->int8_flops += (double)p.m * (double)p.n * (double)p.n;
@@ -453,7 +473,7 @@ extern "C" uvt_status uvt_batch_update(uvt_batch b, const uvt_problem* probs, in
     if (!b || !probs || count != b->count) return UVT_ERR_ARG;
     for (int i = 0; i < count; ++i) {
         const uvt_problem &p = probs[i], &q = b->probs[i];
-        if (p.n != q.n || p.m != q.m || p.prec != q.prec) return UVT_ERR_ARG;
+        if (p.n != q.n || p.m != q.m || p.prec != q.prec || p.route != q.route) return UVT_ERR_ARG;
         uvt_status st = check(p);
         if (st != UVT_OK) return st;
         // the 16-byte load path was fixed at capture time
@@ -486,6 +506,21 @@ extern "C" uvt_status uvt_batch_info(uvt_batch b, double* flops, double* int8_fl
         *fp64_tiles = t;
     }
     if (graph_nodes) *graph_nodes = (int32_t)b->nodes;
+    return UVT_OK;
+}
+
+extern "C" uvt_status uvt_batch_errors(uvt_batch b, double* bound, double* estimate, int32_t count)
+{
+    if (!b || count != b->count) return UVT_ERR_ARG;
+    UVT_HIP(hipSetDevice(b->h->device));
+    UVT_HIP(hipDeviceSynchronize());
+    int k = 0;
+    for (int i = 0; i < count; ++i) {
+        double e[2] = {-1.0, -1.0};  // the problem ran in FP64
+        if (b->int8[i]) UVT_HIP(hipMemcpy(e, b->split_err[k++], sizeof e, hipMemcpyDeviceToHost));
+        if (bound) bound[i] = e[0];
+        if (estimate) estimate[i] = e[1];
+    }
     return UVT_OK;
 }
 

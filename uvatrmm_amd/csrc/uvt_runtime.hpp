@@ -36,18 +36,18 @@ UVT_INTERNAL uvt_status uvt_noalloc_setup(uvt_handle h);
         }                                             \
     } while (0)
 
-uvt_status uvt_handle_workspace(uvt_handle h, size_t bytes, void** p);
-uvt_status uvt_launch_tiles_f64(int32_t cfg, const uvt_tile* dtiles, int64_t ntiles,
+UVT_INTERNAL uvt_status uvt_handle_workspace(uvt_handle h, size_t bytes, void** p);
+UVT_INTERNAL uvt_status uvt_launch_tiles_f64(int32_t cfg, const uvt_tile* dtiles, int64_t ntiles,
                                 const double* A, int64_t lda, const double* B, int64_t ldb,
                                 double* C, int64_t ldc, int64_t m, hipStream_t s);
 // FP64 C = tril(A) * B as one trapezoid walk with the given tile shape,
 // tile coordinates from the workgroup index (no plan, nothing to upload).
-uvt_status uvt_launch_trapezoid_f64(int32_t cfg, int64_t n, int64_t m, const double* A, int64_t lda,
+UVT_INTERNAL uvt_status uvt_launch_trapezoid_f64(int32_t cfg, int64_t n, int64_t m, const double* A, int64_t lda,
                                     const double* B, int64_t ldb, double* C, int64_t ldc,
                                     hipStream_t s);
 // FP64 C[0:M, 0:N] = A[0:M, 0:K] * B[0:K, 0:N] (no triangle), tile coordinates
 // from the workgroup index.
-uvt_status uvt_launch_gemm_f64(int32_t cfg, int64_t M, int64_t N, int64_t K, const double* A,
+UVT_INTERNAL uvt_status uvt_launch_gemm_f64(int32_t cfg, int64_t M, int64_t N, int64_t K, const double* A,
                                int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc,
                                hipStream_t s);
 
@@ -57,9 +57,9 @@ struct uvt_tune_entry {
     int64_t n_max, m_max;
     int cfg;
 };
-bool uvt_tune_load(const char* path, std::vector<uvt_tune_entry>& out);
+UVT_INTERNAL bool uvt_tune_load(const char* path, std::vector<uvt_tune_entry>& out);
 // Shape for (n, m); the built-in default when no entry matches.
-int uvt_tune_pick(const std::vector<uvt_tune_entry>& table, int64_t n, int64_t m);
+UVT_INTERNAL int uvt_tune_pick(const std::vector<uvt_tune_entry>& table, int64_t n, int64_t m);
 
 // Where a split-class problem (error <= 1e-12 acceptable) runs on the int8
 // path rather than in FP64, which is always at least as accurate.  Measured on

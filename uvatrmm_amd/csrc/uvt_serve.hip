@@ -32,6 +32,7 @@
 #include "uvt_internal.h"
 #include "uvt_runtime.hpp"
 #include "uvt_serve_arena.hpp"
+#include "uvt_split.hpp"
 
 namespace {
 
@@ -84,6 +85,7 @@ struct Slot {
     double* dC = nullptr;
     size_t off = SIZE_MAX;  // the request's arena block (p.bytes long)
     uvt_status status = UVT_OK;
+    const char* err = nullptr;  // int8 path: the error statement in the workspace
 };
 
 
@@ -119,13 +121,20 @@ struct uvt_server_s {
     }
 };
 
+// Whether a request's box part runs on the int8 path: split class, measured
+// faster alone at this size, and a size the int8 path supports.
+static bool runs_int8(const uvt_serve_config& cfg, int64_t n, int64_t m, int32_t prec)
+{
+    return prec == UVT_PREC_SPLIT && uvt_split_faster_alone(n, m) && uvt_split_supported(n, cfg.slices);
+}
+
 static size_t request_bytes(const uvt_serve_config& cfg, int64_t n, int64_t m, int32_t prec,
                             bool ab_in_arena, bool c_in_arena)
 {
     size_t b = 0;
     if (ab_in_arena) b += align_up((size_t)n * n * 8) + align_up((size_t)n * m * 8);
     if (c_in_arena) b += align_up((size_t)n * m * 8);
-    if (prec == UVT_PREC_SPLIT && uvt_split_faster_alone(n, m)) {
+    if (runs_int8(cfg, n, m, prec)) {
         size_t w = 0;
         if (uvt_trmm_split_workspace(n, m, cfg.slices, &w) == UVT_OK) b += align_up(w);
     }
@@ -160,12 +169,14 @@ static uvt_status launch(uvt_server s, Slot& sl)
     const double* dA = own_ab ? (const double*)carve((size_t)n * n * 8) : r.A;
     const double* dB = own_ab ? (const double*)carve((size_t)n * m * 8) : r.B;
     double* dC = c_in_arena(r) ? (double*)carve((size_t)n * m * 8) : r.C;
-    const bool int8 = r.prec == UVT_PREC_SPLIT && uvt_split_faster_alone(n, m);
+    const bool int8 = runs_int8(s->cfg, n, m, r.prec);
     size_t wbytes = 0;
     void* work = nullptr;
+    sl.err = nullptr;
     if (int8) {
         if (uvt_trmm_split_workspace(n, m, s->cfg.slices, &wbytes) != UVT_OK) return UVT_ERR_ARG;
         work = carve(wbytes);
+        sl.err = (const char*)work + uvt_split_err_offset(n, m, s->cfg.slices);
     }
     if (cursor > s->arena.at(sl.off) + sl.p.bytes) return UVT_ERR_NOMEM;  // cannot happen
     sl.dC = dC;
@@ -241,6 +252,17 @@ static void scheduler(uvt_server s)
                     sl.status = UVT_ERR_HIP;
                 }
             }
+            // the int8 path's error statement, read before its workspace goes
+            double err[2] = {-1.0, -1.0};
+            if (sl.status == UVT_OK && sl.err) {
+                lk.unlock();
+                const hipError_t c = hipMemcpy(err, sl.err, sizeof err, hipMemcpyDeviceToHost);
+                lk.lock();
+                if (c != hipSuccess) {
+                    (void)hipGetLastError();
+                    sl.status = UVT_ERR_HIP;
+                }
+            }
             const double t = s->now();
             s->arena.free(sl.off, sl.p.bytes);
             sl.off = SIZE_MAX;
@@ -249,9 +271,12 @@ static void scheduler(uvt_server s)
             c.ticket = sl.p.ticket;
             c.tag = sl.p.r.tag;
             c.status = sl.status;
+            c.path = sl.err ? 1 : 0;
             c.t_submit = sl.p.t_submit;
             c.t_admit = sl.t_admit;
             c.t_done = t;
+            c.err_bound = err[0];
+            c.err_estimate = err[1];
             s->done.push_back(c);
             s->completed++;
             if (sl.status == UVT_OK) {
