@@ -63,8 +63,7 @@ constexpr int kMaxMods = 16;
 constexpr int kMaxSlices = 14;  // limb sums of the reconstruction stay below 14 * 256 * 2^40 < 2^52
 constexpr int kTileI = 256;  // int8 tile rows = row-tile granularity of the box/diagonal split
 constexpr int kI8BK = 64;    // k bytes per LDS stage (one v_mfma_i32_16x16x64_i8 step)
-constexpr int kI8S = kI8BK + 16;  // LDS row stride (bytes): conflict-free 16-byte fragment reads
-constexpr int kI8Stage = kTileI * kI8S;  // one B stage
+constexpr int kI8Stage = kTileI * kI8BK;  // one B' stage in LDS: 16 fragments of 1 KiB
 constexpr int kFrag = 1024;       // one MFMA operand fragment: 16 rows x 64 k bytes
 constexpr int kFragSkew = 256;    // extra bytes per 16-row block of A' (see a_tile_stride)
 constexpr int kNonFinite = 0x40000000;    // alpha of a row whose box part holds a NaN / infinity
@@ -130,7 +129,7 @@ struct Desc {
     xd* cpart;                // [count[K_REC]]: sum of squares of C per reconstruction workgroup
     double* err;              // [4]: bound, estimate (relative), ||C||_F, absolute bound
     int8_t* Ar;               // [mod][row tile t][16 blocks of a_tile_stride(t) bytes] (see k_resid_a)
-    int8_t* Br;               // [mod in group][pad_tile(m)][kmax] (B transposed)
+    int8_t* Br;               // [mod in group][k block][column block] fragments (see k_resid_b)
     uint8_t* R;               // [mod][n - 256][m]
     double target;            // row / column 2-norm target before rounding
     double c_max;             // bound on |C'_ij| = |A'_i B'_j|
@@ -464,18 +463,24 @@ __global__ __launch_bounds__(256) void k_resid_a(Desc one, const Desc* table, co
     }
 }
 
-// Br[l][j][k] (B transposed) for k < kmax and the moduli l0 .. l0+nl-1 of the
-// launch: 128 (k) x 64 (j) tiles through LDS.  Thread (j = t%64, kq = t/64)
-// converts k = 4 (kq + 4p) + e of column j (p < 8, e < 4), so it packs whole
-// 4-byte words of the transposed tile; the words go to LDS k-word-major
-// (consecutive lanes, consecutive banks) and come back as 16-byte pieces, 8
-// lanes to each 128-byte row piece of Br (whole lines).  Two LDS buffers, so
-// one barrier per modulus.  Rows m .. pad_tile(m) are zeros, and so are
-// non-finite entries.
+// Residues of B' for k < kmax and the moduli l0 .. l0+nl-1 of the launch, in
+// MFMA operand order like A': for column block cb = j/16 and k block
+// kb = k/64, fragment (kb, cb) is 1024 contiguous bytes at
+// (kb * pad_tile(m)/16 + cb) * 1024, where lane l = j%16 + 16*((k%64)/16)
+// holds k bytes 16*(l/16) .. +16 of column j (the 16 bytes lane l feeds
+// v_mfma_i32_16x16x64_i8 as its B operand).  A product workgroup then moves
+// each fragment from memory to LDS and from LDS to its waves as one
+// contiguous 1 KiB per wave.  A workgroup converts 128 k x 64 columns: thread
+// (j = t%64, kq = t/64) converts k = 4 (kq + 4p) + e of column j (p < 8,
+// e < 4) and packs whole 4-byte words, which go to LDS k-word-major and come
+// back so that wave w writes fragments (k block w%2, column blocks w/2 and
+// w/2 + 2), one lane's 16 bytes each.  Two LDS buffers, so one barrier per
+// modulus.  Columns m .. pad_tile(m) are zeros, and so are non-finite
+// entries.
 __global__ __launch_bounds__(256) void k_resid_b(Desc one, const Desc* table, const int* lut, Mods mods)
 {
     constexpr int kW = 32;          // 4-byte words per column of the tile (128 k)
-    constexpr int kPitch = 64 + 2;  // readback: 8 pieces x 8 rows of a wave on distinct banks
+    constexpr int kPitch = 64 + 4;  // 4 * kPitch = 16 (mod 32) banks: the readback's lanes hit distinct banks
     __shared__ uint32_t t[2][kW][kPitch];  // [buffer][k word][column]
     const Desc d = fetch(one, table, lut);
     const int64_t wb = (int64_t)blockIdx.x - d.start[K_RB];
@@ -494,7 +499,9 @@ __global__ __launch_bounds__(256) void k_resid_b(Desc one, const Desc* table, co
             const double b = jv ? gbl(d.B)[k * d.ldb + j] : 0.0;
             v[p * 4 + e] = fabs(b) <= 1.79769313486231570815e308 ? rint(ldexp(b, be)) : 0.0;
         }
-    const int pr = threadIdx.x >> 3, ps = threadIdx.x & 7;  // readback rows pr, pr + 32; piece ps
+    // readback: wave w, lane = c + 16 q -> column block w/2 (+2), k block w%2
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, c = lane & 15, q = lane >> 4;
+    const int64_t ncb = pad_tile(d.m) / 16;
     const int64_t mod_stride = pad_tile(d.m) * d.kmax;
     for (int l = 0; l < mods.nl; ++l) {
         const int lm = mods.l0 + l;
@@ -514,11 +521,11 @@ __global__ __launch_bounds__(256) void k_resid_b(Desc one, const Desc* table, co
         __syncthreads();
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int row = pr + 32 * h;
-            const v4i x = {(int)tb[4 * ps][row], (int)tb[4 * ps + 1][row], (int)tb[4 * ps + 2][row],
-                           (int)tb[4 * ps + 3][row]};
-            *reinterpret_cast<UVT_GLOBAL v4i*>(gbl(d.Br) + l * mod_stride + (j0 + row) * d.kmax + k0 +
-                                               16 * ps) = x;
+            const int cbl = (w >> 1) + 2 * h, kbl = w & 1;
+            const int col = cbl * 16 + c, kw = kbl * 16 + 4 * q;
+            const v4i x = {(int)tb[kw][col], (int)tb[kw + 1][col], (int)tb[kw + 2][col], (int)tb[kw + 3][col]};
+            const int64_t frag = (k0 / 64 + kbl) * ncb + j0 / 16 + cbl;
+            *reinterpret_cast<UVT_GLOBAL v4i*>(gbl(d.Br) + l * mod_stride + frag * kFrag + lane * 16) = x;
         }
     }
 }
@@ -527,8 +534,10 @@ __global__ __launch_bounds__(256) void k_resid_b(Desc one, const Desc* table, co
 
 // The k loop of one 256 x 256 tile; 8 waves of 64 x 128, each a 4 x 8 block
 // of v_mfma_i32_16x16x64_i8.  A' fragments come straight from memory into
-// registers (fragment order, one coalesced 1 KiB load each); B' panels go
-// through a double-buffered LDS tile of 64 k.  Loads run two k stages ahead:
+// registers (fragment order, one coalesced 1 KiB load each); B' fragments go
+// through a double-buffered LDS stage of 64 k (16 fragments: each wave loads
+// two and stores them as they are, and every wave reads its eight; all
+// contiguous 1 KiB per wave, so no LDS bank conflicts).  Loads run two k stages ahead:
 // the products are bound by memory latency, not bandwidth (one stage of
 // prefetch left every CU waiting on about 48 KiB in flight, measured), so
 // A' has two register sets, even and odd stages, and each fragment slot is
@@ -537,7 +546,7 @@ __global__ __launch_bounds__(256) void k_resid_b(Desc one, const Desc* table, co
 // is unrolled by two so every buffer has a fixed name.  The residue arrays
 // are padded to whole tiles, so no load needs a bounds check.
 __device__ __forceinline__ void i8_tile_loop(const int8_t* __restrict__ Af, int64_t astr,
-                                             const int8_t* __restrict__ B, int64_t ldb_r, int K,
+                                             const int8_t* __restrict__ B, int64_t bstr, int K,
                                              int8_t* smem, v4i (&acc)[4][8])
 {
     // Addresses are a wave-uniform base (scalar registers) plus one 32-bit
@@ -548,26 +557,27 @@ __device__ __forceinline__ void i8_tile_loop(const int8_t* __restrict__ Af, int6
     // this lane's 16 bytes of fragment (row block 4*wm + i, k block kt)
     const int8_t* Awave = Af + (int64_t)(4 * wm) * astr;
     const uint32_t aoff = (uint32_t)lane * 16;
-    // B panel rows tid/4 + 128 q, 16 bytes at (tid%4)*16 of the stage
-    const uint32_t boff = (uint32_t)(tid >> 2) * (uint32_t)ldb_r + (uint32_t)(tid & 3) * 16;
-    const uint32_t soff = (uint32_t)(tid >> 2) * kI8S + (uint32_t)(tid & 3) * 16;
+    // B' fragments 2*wave and 2*wave + 1 of a stage (bstr bytes per k block)
+    const int8_t* Bwave = B + (int64_t)(2 * wave) * kFrag;
+    const uint32_t boff = (uint32_t)lane * 16;
+    const uint32_t soff = (uint32_t)(2 * wave) * kFrag + (uint32_t)lane * 16;
     v4i aE[4], aO[4], bE[2], bO[2];
     auto load_a = [&](int kt, int i) { return ld_global16(Awave + (int64_t)i * astr + (int64_t)kt * kFrag, aoff); };
     auto load_b = [&](int kt, v4i (&d)[2]) {
 #pragma unroll
-        for (int q = 0; q < 2; ++q) d[q] = ld_global16(B + (int64_t)q * 128 * ldb_r + kt * kI8BK, boff);
+        for (int q = 0; q < 2; ++q) d[q] = ld_global16(Bwave + (int64_t)kt * bstr + q * kFrag, boff);
     };
     auto store_b = [&](int buf, const v4i (&d)[2]) {
         int8_t* bs = smem + buf * kI8Stage + soff;
 #pragma unroll
-        for (int q = 0; q < 2; ++q) *reinterpret_cast<v4i*>(bs + q * 128 * kI8S) = d[q];
+        for (int q = 0; q < 2; ++q) *reinterpret_cast<v4i*>(bs + q * kFrag) = d[q];
     };
     // Products of stage kt from LDS buffer buf; refills a for stage kn.
     auto compute = [&](int buf, v4i (&a)[4], int kn) {
-        const int8_t* bs = smem + buf * kI8Stage + (wn * 128 + (lane & 15)) * kI8S + (lane >> 4) * 16;
+        const int8_t* bs = smem + buf * kI8Stage + wn * 8 * kFrag + lane * 16;
         v4i b[8];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) b[j] = *reinterpret_cast<const v4i*>(bs + j * 16 * kI8S);
+        for (int j = 0; j < 8; ++j) b[j] = *reinterpret_cast<const v4i*>(bs + j * kFrag);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
 #pragma unroll
@@ -649,7 +659,7 @@ __global__ __launch_bounds__(512, 1) void k_gemm_i8(Desc one, const Desc* table,
     const int64_t n = UVT_FIELD(n), m = UVT_FIELD(m), kmax = UVT_FIELD(kmax);
     const int lm = mods.l0 + l;  // the modulus
     const int8_t* Af = UVT_FIELD(Ar) + lm * a_mod_bytes(n) + a_tile_base(rt);
-    const int8_t* B = UVT_FIELD(Br) + l * (pad_tile(m) * kmax) + c0 * kmax;
+    const int8_t* B = UVT_FIELD(Br) + l * (pad_tile(m) * kmax) + (c0 / 16) * kFrag;
     UVT_GLOBAL uint8_t* const Rout = gbl(UVT_FIELD(R));
 #undef UVT_FIELD
 
@@ -658,7 +668,7 @@ __global__ __launch_bounds__(512, 1) void k_gemm_i8(Desc one, const Desc* table,
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 8; ++j) acc[i][j] = v4i{0, 0, 0, 0};
-    i8_tile_loop(Af, a_tile_stride(rt), B, kmax, (int)r0, smem, acc);
+    i8_tile_loop(Af, a_tile_stride(rt), B, pad_tile(m) / 16 * kFrag, (int)r0, smem, acc);
 
     // D layout of v_mfma_i32_16x16x64_i8: lane l, element v holds row
     // 4*(l/16) + v, column l%16.
