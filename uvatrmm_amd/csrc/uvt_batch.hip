@@ -5,9 +5,9 @@
 // A batch is captured once into a graph:
 //   zero the work counters (a fill kernel, not a memset node)
 //   -> scales and residues of the split problems
-//   -> fork: main branch: one persistent kernel per FP64 tile shape, each
-//                         draining a longest-first queue of tiles gathered
-//                         from every problem of that shape
+//   -> fork: main branch: one persistent kernel draining one queue of the
+//                         FP64 tiles of every problem, each tile with its
+//                         problem's tile shape, most work first
 //            side branch: the int8 residue products of the split problems
 //   -> join -> the split reconstruction, which adds the integer box part to
 //              the diagonal blocks the FP64 queue stored.
@@ -42,7 +42,7 @@ using namespace uvt;
 namespace {
 
 struct BatchTile {
-    int32_t prob, r0, rows, c0, k_lo, k_hi;
+    int32_t prob, r0, rows, c0, k_lo, k_hi, cfg;  // cfg: UVT_CFG_* tile shape
 };
 
 struct BatchProb {
@@ -53,17 +53,27 @@ struct BatchProb {
     int32_t vec, pad;
 };
 
-// Persistent FP64 tile kernel: every workgroup takes the next tile of the
-// queue (an atomic counter) until the queue is empty, so all waves exit once
-// `ntiles` tiles have been handed out.
-template <int TM, int TN, bool VEC>
-__global__ __launch_bounds__((64 * TileWaves<TM, TN>::M * TileWaves<TM, TN>::N),
-                             (TileWaves<TM, TN>::WAVES_PER_EU))
+// Persistent FP64 tile kernel for every tile shape: every workgroup (8
+// waves, LDS for the largest shape) takes the next tile of the queue (an
+// atomic counter) and runs it with that tile's shape, until the queue is
+// empty, so all waves exit once `ntiles` tiles have been handed out.  One
+// queue for all shapes means one drain at the end of the batch instead of
+// one per shape.  128x128 tiles run as 4 x 2 waves of 32x64, 64x64 tiles as
+// 2 x 4 waves of 32x16, 128x32 tiles as 4 x 2 waves of 32x16 (the B panel
+// has half as many pieces as threads).  The order of every sum does not
+// depend on the wave layout, so a tile gives the same bits as the
+// single-problem kernels' tiles of the same shape.
+constexpr int kBatchThreads = 512;
+template <bool VEC>
+__global__ __launch_bounds__(kBatchThreads, 4)
 void k_batch_f64(const BatchTile* __restrict__ tiles, int ntiles, const BatchProb* __restrict__ probs,
                  int* __restrict__ counter)
 {
-    using W = TileWaves<TM, TN>;
-    __shared__ __attribute__((aligned(16))) double smem[F64Cfg<TM, TN, W::M, W::N>::SMEM_DOUBLES];
+    __shared__ __attribute__((aligned(16))) double smem[F64Cfg<128, 128, 4, 2>::SMEM_DOUBLES];
+    static_assert(F64Cfg<64, 64, 2, 4>::SMEM_DOUBLES <= F64Cfg<128, 128, 4, 2>::SMEM_DOUBLES &&
+                      F64Cfg<128, 64, 4, 2>::SMEM_DOUBLES <= F64Cfg<128, 128, 4, 2>::SMEM_DOUBLES &&
+                      F64Cfg<128, 32, 4, 2>::SMEM_DOUBLES <= F64Cfg<128, 128, 4, 2>::SMEM_DOUBLES,
+                  "LDS sized for the largest shape");
     __shared__ int next;
     for (;;) {
         if (threadIdx.x == 0) next = atomicAdd(counter, 1);
@@ -73,9 +83,21 @@ void k_batch_f64(const BatchTile* __restrict__ tiles, int ntiles, const BatchPro
         if (t >= ntiles) break;
         const BatchTile bt = tiles[t];
         const BatchProb p = probs[bt.prob];
-        f64_tile<TM, TN, VEC, W::M, W::N>(bt.r0, bt.rows, bt.c0, bt.k_lo, bt.k_hi, kTileLower,
-                                          gbl(p.A), p.lda, gbl(p.B), p.ldb, gbl(p.C), p.ldc, p.m,
-                                          smem);
+        // p.pad is always 0, but loaded per tile: the thread index built from
+        // it keeps each shape's offset arithmetic inside its own branch
+        const int tid = (int)threadIdx.x + p.pad;
+        if (bt.cfg == UVT_CFG_64x64)
+            f64_tile<64, 64, VEC, 2, 4>(bt.r0, bt.rows, bt.c0, bt.k_lo, bt.k_hi, kTileLower, gbl(p.A),
+                                        p.lda, gbl(p.B), p.ldb, gbl(p.C), p.ldc, p.m, smem, tid);
+        else if (bt.cfg == UVT_CFG_128x64)
+            f64_tile<128, 64, VEC, 4, 2>(bt.r0, bt.rows, bt.c0, bt.k_lo, bt.k_hi, kTileLower, gbl(p.A),
+                                         p.lda, gbl(p.B), p.ldb, gbl(p.C), p.ldc, p.m, smem, tid);
+        else if (bt.cfg == UVT_CFG_128x32)
+            f64_tile<128, 32, VEC, 4, 2>(bt.r0, bt.rows, bt.c0, bt.k_lo, bt.k_hi, kTileLower, gbl(p.A),
+                                         p.lda, gbl(p.B), p.ldb, gbl(p.C), p.ldc, p.m, smem, tid);
+        else
+            f64_tile<128, 128, VEC, 4, 2>(bt.r0, bt.rows, bt.c0, bt.k_lo, bt.k_hi, kTileLower, gbl(p.A),
+                                          p.lda, gbl(p.B), p.ldb, gbl(p.C), p.ldc, p.m, smem, tid);
         __syncthreads();  // the LDS tile is reused by the next tile
     }
 }
@@ -86,46 +108,24 @@ __global__ void k_zero_ints(int* p, int n)
     if (i < n) p[i] = 0;
 }
 
-template <int TM, int TN>
-int blocks_per_cu()
+// Workgroups of the persistent kernel: every CU filled to its occupancy.
+int persistent_grid(int cus)
 {
     int b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_batch_f64<TM, TN, true>,
-                                                     64 * TileWaves<TM, TN>::M * TileWaves<TM, TN>::N,
-                                                     0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_batch_f64<true>, kBatchThreads, 0) != hipSuccess ||
         b < 1) {
         (void)hipGetLastError();
         b = 1;
     }
-    return b;
+    return cus * b;
 }
 
-// Workgroups of a persistent kernel: every CU filled to its occupancy.
-int persistent_grid(int cfg, int cus)
+void launch_persistent(bool vec, int grid, const BatchTile* tiles, int ntiles, const BatchProb* probs,
+                       int* counter, hipStream_t s)
 {
-    if (cfg == UVT_CFG_64x64) return cus * blocks_per_cu<64, 64>();
-    if (cfg == UVT_CFG_128x32) return cus * blocks_per_cu<128, 32>();
-    return cus * blocks_per_cu<128, 128>();
-}
-
-template <int TM, int TN>
-void launch_cfg(bool vec, dim3 g, const BatchTile* tiles, int ntiles, const BatchProb* probs,
-                int* counter, hipStream_t s)
-{
-    const dim3 block(64 * TileWaves<TM, TN>::M * TileWaves<TM, TN>::N);
-    if (vec)
-        hipLaunchKernelGGL((k_batch_f64<TM, TN, true>), g, block, 0, s, tiles, ntiles, probs, counter);
-    else
-        hipLaunchKernelGGL((k_batch_f64<TM, TN, false>), g, block, 0, s, tiles, ntiles, probs, counter);
-}
-
-void launch_persistent(int cfg, bool vec, int grid, const BatchTile* tiles, int ntiles,
-                       const BatchProb* probs, int* counter, hipStream_t s)
-{
-    const dim3 g((unsigned)std::max(1, std::min(grid, ntiles)));
-    if (cfg == UVT_CFG_64x64) launch_cfg<64, 64>(vec, g, tiles, ntiles, probs, counter, s);
-    else if (cfg == UVT_CFG_128x32) launch_cfg<128, 32>(vec, g, tiles, ntiles, probs, counter, s);
-    else launch_cfg<128, 128>(vec, g, tiles, ntiles, probs, counter, s);
+    const dim3 g((unsigned)std::max(1, std::min(grid, ntiles))), block(kBatchThreads);
+    if (vec) hipLaunchKernelGGL(k_batch_f64<true>, g, block, 0, s, tiles, ntiles, probs, counter);
+    else hipLaunchKernelGGL(k_batch_f64<false>, g, block, 0, s, tiles, ntiles, probs, counter);
 }
 
 // ---- tuning table ------------------------------------------------------------
@@ -201,6 +201,9 @@ namespace {
 // Tiles of the lower trapezoid of one problem.  diag_only: only the 256-row
 // diagonal blocks (k from the start of the row's 256-row tile), the FP64 part
 // of a split problem.
+// With 128x128 tiles the last column block gets a tile as narrow as it
+// allows (32, 64 or 128 columns), so a problem issues at most 31 columns of
+// padding products instead of up to 127.
 void trapezoid_tiles(int prob, int64_t n, int64_t m, int cfg, std::vector<BatchTile>& out,
                      bool diag_only = false)
 {
@@ -209,16 +212,23 @@ void trapezoid_tiles(int prob, int64_t n, int64_t m, int cfg, std::vector<BatchT
         for (int64_t c0 = 0; c0 < m; c0 += tn) {
             const int64_t rows = std::min(tm, n - r0);
             const int64_t k_lo = diag_only ? (r0 / 256) * 256 : 0;
+            int c = cfg;
+            if (cfg == UVT_CFG_128x128 && m - c0 <= 64) {
+                c = m - c0 <= 32 ? UVT_CFG_128x32 : UVT_CFG_128x64;
+            }
             out.push_back(BatchTile{prob, (int32_t)r0, (int32_t)rows, (int32_t)c0, (int32_t)k_lo,
-                                    (int32_t)(r0 + rows)});
+                                    (int32_t)(r0 + rows), c});
         }
 }
 
+// Most work first (depth times tile area), so the queue drains with its
+// smallest tiles.
 void sort_longest_first(std::vector<BatchTile>& v)
 {
-    std::stable_sort(v.begin(), v.end(), [](const BatchTile& a, const BatchTile& b) {
-        return a.k_hi - a.k_lo > b.k_hi - b.k_lo;
-    });
+    auto work = [](const BatchTile& t) {
+        return (int64_t)(t.k_hi - t.k_lo) * uvt_cfg_tile_m(t.cfg) * uvt_cfg_tile_n(t.cfg);
+    };
+    std::stable_sort(v.begin(), v.end(), [&](const BatchTile& a, const BatchTile& b) { return work(a) > work(b); });
 }
 
 // Split-class problems whose integer box part is too small to pay for its
@@ -233,9 +243,9 @@ struct uvt_batch_s {
     // FP64 part
     std::vector<int> f64_index;        // problem -> slot in the FP64 table
     std::vector<BatchProb> f64_probs;  // host image
-    // one queue per (tile shape, 16-byte load path): index cfg * 2 + vec
-    std::vector<BatchTile> tiles[2 * UVT_NUM_CFGS];
-    BatchTile* d_tiles[2 * UVT_NUM_CFGS] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    // one queue of every tile shape per 16-byte load path: index vec
+    std::vector<BatchTile> tiles[2];
+    BatchTile* d_tiles[2] = {nullptr, nullptr};
     BatchProb* d_probs = nullptr;
     int* d_counters = nullptr;
     // split part (problems that run the integer stages)
@@ -359,7 +369,7 @@ extern "C" uvt_status uvt_batch_create(uvt_handle h, const uvt_problem* probs, i
         b->f64_probs.push_back(f64_desc(p));
         const int cfg = table.empty() ? uvt_default_cfg(p.n, p.m) : uvt_tune_pick(table, p.n, p.m);
         const bool use_int8 = routes_to_int8(p, slices);
-        trapezoid_tiles(slot, p.n, p.m, cfg, b->tiles[cfg * 2 + b->f64_probs.back().vec], use_int8);
+        trapezoid_tiles(slot, p.n, p.m, cfg, b->tiles[b->f64_probs.back().vec], use_int8);
         b->int8.push_back(use_int8);
         if (use_int8) b->int8_flops += (double)p.m * (double)p.n * (double)p.n;
         if (use_int8) b->sp.push_back(uvt_split_problem{p.A, p.lda, p.B, p.ldb, p.C, p.ldc, p.n, p.m});
@@ -380,7 +390,7 @@ extern "C" uvt_status uvt_batch_create(uvt_handle h, const uvt_problem* probs, i
         }                                \
     } while (0)
     // FP64 buffers
-    for (int c = 0; c < 2 * UVT_NUM_CFGS; ++c)
+    for (int c = 0; c < 2; ++c)
         if (!b->tiles[c].empty()) {
             BATCH_HIP(hipMalloc(&b->d_tiles[c], b->tiles[c].size() * sizeof(BatchTile)));
             BATCH_HIP(hipMemcpy(b->d_tiles[c], b->tiles[c].data(), b->tiles[c].size() * sizeof(BatchTile),
@@ -415,8 +425,7 @@ extern "C" uvt_status uvt_batch_create(uvt_handle h, const uvt_problem* probs, i
         }
     }
     // capture (occupancy queries first: nothing but stream work inside the capture)
-    int grids[UVT_NUM_CFGS];
-    for (int c = 0; c < UVT_NUM_CFGS; ++c) grids[c] = persistent_grid(c, h->cus);
+    const int grid = persistent_grid(h->cus);
     BATCH_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
     BATCH_HIP(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
     BATCH_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
@@ -436,10 +445,10 @@ extern "C" uvt_status uvt_batch_create(uvt_handle h, const uvt_problem* probs, i
         st = uvt_split_batch_launch(b->sp.data(), (int)b->sp.size(), slices, b->d_split_work,
                                     b->d_split_desc, b->d_split_lut, b->split_totals, b->split_lut_off,
                                     b->side, UVT_SPLIT_GEMM);
-    for (int q = 0; q < 2 * UVT_NUM_CFGS; ++q)
+    for (int q = 0; q < 2; ++q)
         if (!b->tiles[q].empty())
-            launch_persistent(q / 2, q & 1, grids[q / 2], b->d_tiles[q], (int)b->tiles[q].size(),
-                              b->d_probs, b->d_counters + 16 * q, b->cap);
+            launch_persistent(q == 1, grid, b->d_tiles[q], (int)b->tiles[q].size(), b->d_probs,
+                              b->d_counters + 16 * q, b->cap);
     BATCH_HIP(hipEventRecord(b->join, b->side));
     BATCH_HIP(hipStreamWaitEvent(b->cap, b->join, 0));
     // the reconstruction adds to the diagonal blocks the FP64 queue stored
@@ -599,8 +608,7 @@ extern "C" uvt_status uvt_autotune_f64(uvt_handle h, const char* out_path)
                 break;
             }
             int best_cfg = uvt_default_cfg(n, m);
-            int grids[UVT_NUM_CFGS];
-            for (int c = 0; c < UVT_NUM_CFGS; ++c) grids[c] = persistent_grid(c, h->cus);
+            const int grid = persistent_grid(h->cus);
             float best_ms = 1e30f;
             float per_cfg[UVT_NUM_CFGS] = {0, 0, 0};
             for (int cfg = 0; cfg < UVT_NUM_CFGS; ++cfg) {
@@ -618,7 +626,7 @@ extern "C" uvt_status uvt_autotune_f64(uvt_handle h, const char* out_path)
                 for (int rep = 0; rep < 4; ++rep) {
                     hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, counter, 16);
                     (void)hipEventRecord(e0, s);
-                    launch_persistent(cfg, true, grids[cfg], dtiles, (int)tiles.size(), dprobs, counter, s);
+                    launch_persistent(true, grid, dtiles, (int)tiles.size(), dprobs, counter, s);
                     (void)hipEventRecord(e1, s);
                     float ms_ = 0.f;
                     if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms_, e0, e1) != hipSuccess) {

@@ -67,10 +67,14 @@ struct F64Cfg {
     static constexpr int A_ELEMS = TM * SA;
     static constexpr int B_ELEMS = BK * LDB_S;
     static constexpr int SMEM_DOUBLES = 2 * (A_ELEMS + B_ELEMS);
-    static constexpr int A_VEC = TM * BK / 2 / NT;  // 2-double loads per thread
-    static constexpr int B_VEC = TN * BK / 2 / NT;
-    static_assert(A_VEC >= 1 && B_VEC >= 1, "tile too small for the thread count");
-    static_assert(TM * BK / 2 % NT == 0 && TN * BK / 2 % NT == 0, "uneven panel split");
+    // 2-double pieces of a panel, and per thread (a panel smaller than the
+    // workgroup leaves the last threads idle for it: B_PART / A_PART)
+    static constexpr int A_PIECES = TM * BK / 2, B_PIECES = TN * BK / 2;
+    static constexpr int A_VEC = (A_PIECES + NT - 1) / NT;
+    static constexpr int B_VEC = (B_PIECES + NT - 1) / NT;
+    static constexpr bool A_PART = A_PIECES < NT, B_PART = B_PIECES < NT;
+    static_assert(A_PIECES % NT == 0 || A_PART, "uneven A panel split");
+    static_assert(B_PIECES % NT == 0 || B_PART, "uneven B panel split");
     static_assert(FM >= 1 && FN >= 1, "wave tile below 16x16");
 };
 
@@ -80,12 +84,17 @@ __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int
                                          const UVT_GLOBAL double* __restrict__ A, int64_t lda,
                                          const UVT_GLOBAL double* __restrict__ B, int64_t ldb,
                                          UVT_GLOBAL double* __restrict__ C, int64_t ldc, int64_t m,
-                                         double* smem)
+                                         double* smem, int thread = -1)
 {
     using G = F64Cfg<TM, TN, WAVES_M, WAVES_N, BK_>;
     constexpr int BK = G::BK, NT = G::NT;
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
+    // thread: the caller's thread index when it must stay opaque to the
+    // compiler (a persistent kernel running several tile shapes, whose
+    // per-thread offsets would otherwise be hoisted out of its tile loop for
+    // every shape at once and spilled); -1: threadIdx.x
+    const int tid = thread >= 0 ? thread : (int)threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: branches on it stay scalar
     const int wm = wave / WAVES_N, wn = wave % WAVES_N;
     const bool lower = (flags & kTileLower) != 0;
     double* As = smem;                    // [2][TM][SA]
@@ -113,6 +122,7 @@ __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int
 #pragma unroll
         for (int q = 0; q < G::A_VEC; ++q) {
             const int v = tid + q * NT;
+            if (G::A_PART && v >= G::A_PIECES) break;
             const int row = v / (BK / 2), kp = (v % (BK / 2)) * 2;
             const int gk = k0 + kp, gr = r0 + row;
             const UVT_GLOBAL double* p = A + (int64_t)gr * lda + gk;
@@ -129,6 +139,7 @@ __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int
 #pragma unroll
         for (int q = 0; q < G::B_VEC; ++q) {
             const int v = tid + q * NT;
+            if (G::B_PART && v >= G::B_PIECES) break;
             const int kr = v / (TN / 2), col = (v % (TN / 2)) * 2;
             const int gk = k0 + kr;
             const UVT_GLOBAL double* p = B + (int64_t)gk * ldb + (c0 + col);
@@ -150,12 +161,14 @@ __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int
 #pragma unroll
         for (int q = 0; q < G::A_VEC; ++q) {
             const int v = tid + q * NT;
+            if (G::A_PART && v >= G::A_PIECES) break;
             const int row = v / (BK / 2), kp = (v % (BK / 2)) * 2;
             *reinterpret_cast<dbl2*>(&as[row * G::SA + kp]) = ra[q];
         }
 #pragma unroll
         for (int q = 0; q < G::B_VEC; ++q) {
             const int v = tid + q * NT;
+            if (G::B_PART && v >= G::B_PIECES) break;
             const int kr = v / (TN / 2), col = (v % (TN / 2)) * 2;
             *reinterpret_cast<dbl2*>(&bs[kr * G::LDB_S + col]) = rb[q];
         }
@@ -222,10 +235,29 @@ __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int
             compute(kt & 1, true);
             __syncthreads();
         }
-        for (; kt < nk; ++kt) {
+        // In a lower tile, panels that start past the last of this wave's
+        // rows meet only masked (zero) A values: the wave runs them without
+        // products (the second loop: it only stores the next panel), which
+        // leaves the CU's matrix units to the other waves.  The panel count
+        // is uniform over the wave.  Skipping adds of exact zeros changes no
+        // result (the sign of a zero aside).
+        int nk_w = nk;
+        if (lower) {
+            const int last = r0 + wm * G::WM + G::WM - 1;  // last row of this wave
+            nk_w = last >= k_start ? (last - k_start) / BK + 1 : 0;
+            nk_w = nk_w < kt ? kt : (nk_w > nk ? nk : nk_w);
+        }
+        for (; kt < nk_w; ++kt) {
             const bool more = kt + 1 < nk;
             if (more) load(k_start + (kt + 1) * BK, masked);
             compute(kt & 1, more);
+            __syncthreads();
+        }
+        for (; kt < nk; ++kt) {
+            if (kt + 1 < nk) {
+                load(k_start + (kt + 1) * BK, masked);
+                store((kt & 1) ^ 1);
+            }
             __syncthreads();
         }
     }

@@ -16,6 +16,9 @@ extern "C" {
 
 /* FP64 tile kernel configurations (rows x columns of one workgroup tile). */
 enum { UVT_CFG_128x128 = 0, UVT_CFG_64x64 = 1, UVT_CFG_128x32 = 2, UVT_NUM_CFGS = 3 };
+/* 128x64 tiles: only the last column block of a batch problem whose columns
+ * end 33..64 past a multiple of 128 (not a tuning choice). */
+enum { UVT_CFG_128x64 = 3 };
 UVT_INTERNAL int32_t uvt_cfg_tile_m(int32_t cfg);
 UVT_INTERNAL int32_t uvt_cfg_tile_n(int32_t cfg);
 

@@ -36,7 +36,7 @@ const char *uvt_version(void) { return "uvatrmm-mi355x 0.1 (gfx950)"; }
 int32_t uvt_cfg_tile_m(int32_t cfg) { return cfg == UVT_CFG_64x64 ? 64 : 128; }
 int32_t uvt_cfg_tile_n(int32_t cfg)
 {
-    return cfg == UVT_CFG_64x64 ? 64 : cfg == UVT_CFG_128x32 ? 32 : 128;
+    return cfg == UVT_CFG_64x64 || cfg == UVT_CFG_128x64 ? 64 : cfg == UVT_CFG_128x32 ? 32 : 128;
 }
 
 /* Narrow B: a 128x32 tile wastes no columns; small or narrow problems: 64x64
