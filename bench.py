@@ -20,7 +20,10 @@ with MKL, sequential, one thread per rank) replayed on the host cores for a
 bounded column sample of the same matrices, median of 5 runs, for this
 workload and the reference's N=2048 and N=16384 configurations.  The same
 leg computes sampled rows of the exact product (double-double sums) and
-reports the error of the FP64 and split results on them.
+reports the error of the FP64 and split results on them, and times the
+ScaLAPACK pdtrmm comparator when its build is present.
+--vendor-ceiling adds the vendor int8 and FP64 GEMMs (torch: hipBLASLt /
+rocBLAS) timed on the same GPU in a child process, as practical ceilings.
 """
 import argparse
 import ctypes
@@ -39,11 +42,11 @@ from uvatrmm_amd import workloads  # noqa: E402  (seeded workload definitions)
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
 # HBM bytes per launch of the FP64 tile kernel at N=32768, M=8192 (balanced,
 # trapezoid, 16 bands), from two rocprofv3 counter runs of this script:
-# 2.0 x FETCH_SIZE (93,084,523 KiB; the factor is what FETCH_SIZE reads for
+# 2.0 x FETCH_SIZE (92,351,490 KiB; the factor is what FETCH_SIZE reads for
 # k_row_abs_sums streaming exactly 2 GiB in 128-byte requests, the request
-# size of the tile kernel's 128-byte row segments) + WRITE_SIZE (2,129,792
-# KiB).  Kept per workload; other workloads report null.
-MEASURED_TRAFFIC = {(32768, 8192, 16, "balanced", "trapezoid"): 2.0 * 93084523 * 1024 + 2129792 * 1024}
+# size of the tile kernel's 128-byte row segments) + WRITE_SIZE (2,097,152
+# KiB, exactly C).  Kept per workload; other workloads report null.
+MEASURED_TRAFFIC = {(32768, 8192, 16, "balanced", "trapezoid"): 2.0 * 92351490 * 1024 + 2097152 * 1024}
 
 
 def log(*a):
@@ -385,6 +388,43 @@ def serve_leg(args, h, uvt):
     return out
 
 
+def vendor_leg(n=16384):
+    """Sustained rates of the vendor GEMMs on this GPU through torch
+    (hipBLASLt / rocBLAS), as practical ceilings beside the published peaks:
+    int8 x int8 -> int32 (torch._int_mm, B column-major, the faster of its two
+    layouts) and an FP64 matmul, square n = 16384, HIP events on torch's
+    stream.  Opt-in: the first torch import on a fresh machine is slow."""
+    import torch
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    dev = "cuda:0"
+    a = torch.randint(-128, 128, (n, n), dtype=torch.int8, device=dev)
+    b = torch.randint(-128, 128, (n, n), dtype=torch.int8, device=dev).t().contiguous().t()
+    ms_i8 = timed(lambda: torch._int_mm(a, b), 10)
+    del a, b
+    a = torch.randn(n, n, dtype=torch.float64, device=dev)
+    b = torch.randn(n, n, dtype=torch.float64, device=dev)
+    ms_f64 = timed(lambda: a @ b, 3)
+    del a, b
+    torch.cuda.empty_cache()
+    ops = 2.0 * n ** 3
+    out = {"n": n, "int8_ms": round(ms_i8, 4), "int8_tops": round(ops / (ms_i8 * 1e-3) * 1e-12, 1),
+           "fp64_ms": round(ms_f64, 4), "fp64_tflops": round(ops / (ms_f64 * 1e-3) * 1e-12, 2),
+           "via": "torch._int_mm and torch.matmul (hipBLASLt / rocBLAS)"}
+    log(f"vendor GEMMs at n={n}: int8 {out['int8_tops']} TOP/s, FP64 {out['fp64_tflops']} TFLOP/s")
+    return out
+
+
 def configs_leg(args, h, uvt):
     """The reference's other configurations, FP64: N=2048 (M=N, reg, boxes,
     P=4) and N=16384 (M=N, balanced, trapezoid, P=16), seeded inputs."""
@@ -461,7 +501,13 @@ def main():
     ap.add_argument("--serve-max-big", type=int, default=1,
                     help="large requests in flight at once (0 = no cap)")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--vendor-ceiling", action="store_true",
+                    help="also time the vendor int8 and FP64 GEMMs (torch) at n=16384")
+    ap.add_argument("--vendor-only", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.vendor_only:
+        print(json.dumps(vendor_leg()), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the TRMM path shards by columns of B with no "
               "reduction, and no multi-GPU run is part of this build.", file=sys.stderr)
@@ -520,6 +566,18 @@ def main():
     other = configs_leg(args, h, uvt) if not args.no_configs else None
 
     serve = serve_leg(args, h, uvt) if args.serve_requests > 0 else None
+    vendor = None
+    if args.vendor_ceiling:
+        # a child process: torch brings its own HIP runtime, which does not
+        # share a process with the one this library was built against
+        import subprocess
+        log("vendor GEMMs: timing in a child process")
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--vendor-only"],
+                           capture_output=True, text=True, timeout=600)
+        if r.returncode == 0 and r.stdout.strip():
+            vendor = json.loads(r.stdout.strip().splitlines()[-1])
+        else:
+            log("vendor GEMM timing failed:", r.stderr.strip()[-400:])
 
     cpu = None
     if not args.no_cpu:
@@ -573,6 +631,7 @@ def main():
         "batch_forced_split": batch_forced,
         "other_configs": other,
         "serve": serve,
+        "vendor_gemm": vendor,
         "norm": norm,
     }
     print(json.dumps(out), flush=True)
