@@ -103,8 +103,9 @@ def scalapack_leg(args, C_gpu_cols, ranks):
     out = []
     for n, m, cmp_gpu in ((args.n, 1024, True), (2048, 2048, False), (16384, 2048, False)):
         pr, pc = grids[p]
+        # the sample's B is the first m columns of the workload's own B
         cmd = [mpiexec, "-n", str(p), exe, str(n), str(m), "256", str(pr), str(pc), str(CPU_RUNS),
-               str(args.seed), str(args.spread)]
+               str(args.seed), str(args.spread), str(args.m if cmp_gpu else m)]
         env = dict(os.environ, OMP_NUM_THREADS="1", MKL_NUM_THREADS="1")
         r = subprocess.run(cmd, capture_output=True, text=True, timeout=600, env=env)
         if r.returncode != 0:

@@ -11,9 +11,12 @@
  * product is repeated `reps` times on fresh copies of B and timed from a
  * barrier to a barrier, and rank 0 prints one line
  *   n m nb nprow npcol ranks median_s min_s max_s frobenius_norm_of_C
- * (the norm lets the caller compare the result with the GPU's).
+ * (the norm lets the caller compare the result with the GPU's).  B is the
+ * first m columns of the seeded n x mgen matrix (mgen >= m, default m: the
+ * generator indexes element (i, j) as i * mgen + j), so a column sample of a
+ * wider workload gets exactly that workload's values.
  *
- *   mpiexec -n P ./scalapack_trmm.x n m nb nprow npcol reps seed spread
+ *   mpiexec -n P ./scalapack_trmm.x n m nb nprow npcol reps seed spread [mgen]
  */
 #include <math.h>
 #include <mpi.h>
@@ -51,8 +54,8 @@ int main(int argc, char **argv)
     int rank = 0, size = 1;
     MPI_Comm_rank(MPI_COMM_WORLD, &rank);
     MPI_Comm_size(MPI_COMM_WORLD, &size);
-    if (argc != 9) {
-        if (!rank) fprintf(stderr, "usage: %s n m nb nprow npcol reps seed spread\n", argv[0]);
+    if (argc != 9 && argc != 10) {
+        if (!rank) fprintf(stderr, "usage: %s n m nb nprow npcol reps seed spread [mgen]\n", argv[0]);
         MPI_Finalize();
         return 2;
     }
@@ -60,7 +63,8 @@ int main(int argc, char **argv)
     int nprow = atoi(argv[4]), npcol = atoi(argv[5]), reps = atoi(argv[6]);
     const uint64_t seed = strtoull(argv[7], NULL, 10);
     const int spread = atoi(argv[8]);
-    if (nprow * npcol != size || n < 1 || m < 1 || nb < 1 || reps < 1) {
+    const int64_t mgen = argc == 10 ? atoll(argv[9]) : m;
+    if (nprow * npcol != size || n < 1 || m < 1 || nb < 1 || reps < 1 || mgen < m) {
         if (!rank) fprintf(stderr, "grid %dx%d does not match %d ranks, or bad sizes\n", nprow, npcol, size);
         MPI_Finalize();
         return 2;
@@ -94,7 +98,7 @@ int main(int argc, char **argv)
         const int64_t J = global_of(c, nb, mycol, npcol);
         for (int64_t r = 0; r < lr; r++) {
             const int64_t I = global_of(r, nb, myrow, nprow);
-            B0[c * ld + r] = orc_seeded_value(seed, 2u, (uint64_t)(I * m + J), spread, 0);
+            B0[c * ld + r] = orc_seeded_value(seed, 2u, (uint64_t)(I * mgen + J), spread, 0);
         }
     }
     int descA[9], descB[9];
