@@ -449,7 +449,7 @@ def configs_leg(args, h, uvt):
         log(f"config N={n} {mp} {shape} P={p}: {ms:.3f} ms, {tf:.2f} TFLOP/s")
         out.append({"workload": f"trmm_f64 N=M={n} {mp} {shape} P={p}", "ms": round(ms, 4),
                     "tflops": round(tf, 4), "frac": round(tf / FP64_PEAK_TFLOPS, 4),
-                    "launches": plan.phases})
+                    "phases": plan.phases})
         plan.close()
         for b in (dA, dB, dC):
             b.free()

@@ -134,9 +134,19 @@ int32_t uvt_device_cus(uvt_handle h);
  * 16-deep k panels of B, so a NaN or infinity in B[k][j] also reaches the
  * rows above k in k's 128-row tile through A's structural zeros (0 * Inf =
  * NaN), where a BLAS dtrmm would leave them finite; finite inputs are not
- * affected. */
+ * affected.  On its first use on a device the plan allocates its launch
+ * records there, and, for a trapezoid plan of at least 2e9 flops whose
+ * deepest tile is 1.5x or more an even share of the work per workgroup slot
+ * (narrow B: few tiles), up to 64 MiB of partial tiles: its long reductions
+ * run in pieces at once, and the pieces after the first are added to C in
+ * k order (results repeat bit for bit).  Those belong to the plan: calls
+ * with one plan on different streams must be serialized. */
 uvt_status uvt_trmm_f64(uvt_handle h, uvt_plan plan, const double *dA, int64_t lda,
                         const double *dB, int64_t ldb, double *dC, int64_t ldc, void *stream);
+/* How the plan launches on the handle's device (placing its device copies
+ * there if needed): workgroups of its tile launches, and partial tiles
+ * (0 when every tile writes C directly).  Either pointer may be NULL. */
+uvt_status uvt_plan_launch_info(uvt_handle h, uvt_plan plan, int64_t *workgroups, int64_t *partial_tiles);
 
 /* Split-precision C ~= tril(A) * B: the box below each 256-row diagonal
  * block as exact int8 products of residues modulo `slices` coprime moduli

@@ -217,6 +217,45 @@ def test_boxes_equals_trapezoid_and_repeatable(gpu, uvt, oracle):
     np.testing.assert_array_equal(c2, c3)
 
 
+@pytest.mark.parametrize("n,m,p,mapping,shape,cut", [
+    (9000, 250, 1, "balanced", "trapezoid", True),     # narrow B: the deepest tile 2.7x a slot's share
+    (12000, 128, 4, "reg", "trapezoid", True),         # 4x
+    (2048, 2048, 4, "reg", "trapezoid", False),        # 1.45x: whole tiles
+    (4096, 4096, 1, "reg", "trapezoid", False),        # 0.97x
+    (2048, 2048, 4, "reg", "boxes", False),            # the reference's N=2048 configuration
+])
+def test_partial_tiles(gpu, uvt, oracle, n, m, p, mapping, shape, cut):
+    """Plans whose long reductions run in pieces, the pieces after the first
+    added to C afterwards in k order (narrow B, few tiles), and plans that
+    keep whole tiles: against exact double-double rows, and the same bits on
+    every run."""
+    plan = uvt.Plan(n, m, p, mapping, shape)
+    wg, parts = gpu.plan_launch_info(plan)
+    plan.close()
+    say("launch", (n, m, p, mapping, shape), "workgroups", wg, "partial tiles", parts)
+    assert (parts > 0) == cut
+    A, B = oracle.seeded_A(n, 31, 1), oracle.seeded_B(n, m, 31, 1)
+    c1 = run_trmm(gpu, uvt, A, B, p, mapping, shape)
+    c2 = run_trmm(gpu, uvt, A, B, p, mapping, shape)
+    np.testing.assert_array_equal(c1, c2)
+    rng = np.random.default_rng(n + m)
+    rows = np.unique(np.concatenate([[0, 1, 63, 64, 127, 128, n // 2, n - 2, n - 1],
+                                     np.arange(0, n, 512), rng.integers(0, n, 24)]))
+    err = rel_fro(c1[rows], oracle.trmm_rows_dd(A, B, rows))
+    say("partial tiles", (n, m, p, mapping, shape), "err", err)
+    assert err <= 1e-13
+    if shape == "boxes":
+        c3 = run_trmm(gpu, uvt, A, B, p, mapping, "trapezoid")
+        assert rel_fro(c1, c3) <= 1e-14
+
+
+def test_headline_plan_keeps_whole_tiles(gpu, uvt):
+    plan = uvt.Plan(32768, 8192, 16, "balanced", "trapezoid")
+    wg, parts = gpu.plan_launch_info(plan)
+    plan.close()
+    assert (wg, parts) == (256 * 64, 0)
+
+
 def test_large_row_sampled(gpu, uvt, oracle):
     """N=16384 balanced/trapezoid over 16 bands (the single-GPU reference
     configuration's shape) with B cut to 512 columns; rows checked against

@@ -98,6 +98,7 @@ _SIGS = {
     "uvt_destroy": (i32, [vp]),
     "uvt_device_cus": (i32, [vp]),
     "uvt_trmm_f64": (i32, [vp, vp, vp, i64, vp, i64, vp, i64, vp]),
+    "uvt_plan_launch_info": (i32, [vp, vp, _c.POINTER(i64), _c.POINTER(i64)]),
     "uvt_trmm_split_workspace": (i32, [i64, i64, i32, _c.POINTER(sz)]),
     "uvt_trmm_split": (i32, [vp, i64, i64, i32, vp, i64, vp, i64, vp, i64, vp, sz, vp]),
     "uvt_trmm_split_error": (i32, [vp, i64, i64, i32, vp, sz, _c.POINTER(f64), _c.POINTER(f64), vp]),
@@ -307,6 +308,12 @@ class Handle:
     def trmm_f64(self, plan, A, lda, B, ldb, C, ldc):
         check(lib().uvt_trmm_f64(self.ptr, plan.ptr, A.ptr, lda, B.ptr, ldb, C.ptr, ldc,
                                  self.stream), "trmm_f64")
+
+    def plan_launch_info(self, plan):
+        """(workgroups, partial tiles) of the plan's launches on this device."""
+        w, p = i64(), i64()
+        check(lib().uvt_plan_launch_info(self.ptr, plan.ptr, _c.byref(w), _c.byref(p)), "plan_launch_info")
+        return w.value, p.value
 
     def split_workspace(self, n, m, slices):
         b = sz()

@@ -5,6 +5,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "uvt_f64_tile.hpp"
 #include "uvt_internal.h"
 #include "uvt_runtime.hpp"
@@ -17,18 +20,52 @@ using namespace uvt;
 
 namespace {  // kernels: internal to this file
 
+// One workgroup of a plan launch: a plan tile (or a piece of its k range)
+// and where its product goes: C (part < 0), or partial tile `part`, which a
+// later k_add_parts launch adds to C.
+struct LaunchTile {
+    uvt_tile t;
+    int32_t part;
+};
+// C[r0 + r][c0 + c] += partial tiles first, first + 1, ..., first + count - 1,
+// in that order, for rows r < rows of a TM x TN tile.
+struct PartSum {
+    int32_t r0, rows, c0, first, count;
+};
+
 template <int TM, int TN, bool VEC>
 __global__ __launch_bounds__((64 * TileWaves<TM, TN>::M * TileWaves<TM, TN>::N),
                              (TileWaves<TM, TN>::WAVES_PER_EU))
-void k_trmm_tiles_f64(const uvt_tile* __restrict__ tiles, const double* __restrict__ A, int64_t lda,
+void k_trmm_tiles_f64(const LaunchTile* __restrict__ tiles, const double* __restrict__ A, int64_t lda,
                       const double* __restrict__ B, int64_t ldb, double* __restrict__ C,
-                      int64_t ldc, int64_t m)
+                      int64_t ldc, int64_t m, double* __restrict__ parts)
 {
     using W = TileWaves<TM, TN>;
     __shared__ __attribute__((aligned(16))) double smem[F64Cfg<TM, TN, W::M, W::N>::SMEM_DOUBLES];
-    const uvt_tile t = tiles[blockIdx.x];
+    const LaunchTile lt = tiles[blockIdx.x];
+    const uvt_tile& t = lt.t;
+    UVT_GLOBAL double* part = lt.part >= 0 ? gbl(parts) + (int64_t)lt.part * (TM * TN) : nullptr;
     f64_tile<TM, TN, VEC, W::M, W::N>(t.r0, t.rows, t.c0, t.k_lo, t.k_hi, t.flags, gbl(A), lda,
-                                      gbl(B), ldb, gbl(C), ldc, m, smem);
+                                      gbl(B), ldb, gbl(C), ldc, m, smem, -1, part);
+}
+
+// Adds the partial tiles of each record to C in the record's order (one
+// workgroup per record; records cover disjoint parts of C), so the result
+// does not depend on which workgroup finished first.
+__global__ __launch_bounds__(256) void k_add_parts(const PartSum* __restrict__ sums, int tn, int tile,
+                                                   const double* __restrict__ parts, double* __restrict__ C,
+                                                   int64_t ldc, int64_t m)
+{
+    const PartSum ps = sums[blockIdx.x];
+    const UVT_GLOBAL double* P = gbl(parts) + (int64_t)ps.first * tile;
+    for (int e = threadIdx.x; e < ps.rows * tn; e += 256) {
+        const int r = e / tn, c = e % tn;
+        if ((int64_t)ps.c0 + c >= m) continue;
+        UVT_GLOBAL double* p = gbl(C) + (int64_t)(ps.r0 + r) * ldc + (ps.c0 + c);
+        double x = *p;
+        for (int q = 0; q < ps.count; ++q) x += P[(int64_t)q * tile + e];
+        *p = x;
+    }
 }
 
 // The trapezoid walk without a tile list: workgroup b takes row tile
@@ -205,32 +242,128 @@ uvt_status uvt_handle_workspace(uvt_handle h, size_t bytes, void** p)
 extern "C" void uvt_plan_release_device(uvt_plan plan)
 {
     if (!plan) return;
-    for (int ph = 0; ph < 2; ++ph) {
-        if (plan->dev_tiles[ph]) {
+    void** bufs[4] = {&plan->dev_launch[0], &plan->dev_launch[1], &plan->dev_sums, &plan->dev_parts};
+    for (void** b : bufs) {
+        if (*b) {
             (void)hipSetDevice(plan->dev_id);
-            (void)hipFree(plan->dev_tiles[ph]);
-            plan->dev_tiles[ph] = nullptr;
+            (void)hipFree(*b);
+            *b = nullptr;
         }
     }
+    plan->nlaunch[0] = plan->nlaunch[1] = plan->nsums = plan->nparts = 0;
     plan->dev_id = -1;
 }
+
+namespace {
+
+constexpr size_t kMaxPartBytes = (size_t)64 << 20;  // partial tiles of one plan
+// Below this many flops a plan's launch time, not one tile's depth, bounds
+// it, and the plan keeps whole tiles.
+constexpr double kSplitMinFlops = 2e9;
+
+int blocks_per_cu(int32_t cfg)
+{
+    if (cfg == UVT_CFG_64x64) return TileWaves<64, 64>::BLOCKS_PER_CU;
+    if (cfg == UVT_CFG_128x32) return TileWaves<128, 32>::BLOCKS_PER_CU;
+    return TileWaves<128, 128>::BLOCKS_PER_CU;
+}
+
+// The launches of a plan on a GPU of `cus` compute units: its phases, one
+// launch each, with whole tiles, except for a trapezoid plan whose longest
+// reduction is much longer than an even share of the work would give a
+// workgroup slot (narrow B, few tiles: the launch then lasts as long as its
+// deepest tile).  Such a plan cuts every reduction longer than about half
+// that share, kc, into pieces that run at once: the first piece writes C,
+// the others write partial tiles, which k_add_parts adds to C afterwards in
+// k order.  Measured at 9000 x 250 (the deepest tile 2.7x the share): 0.73
+// -> 0.45 ms; at 4096 x 4096 (0.97x) cutting cost 7 %, and between 0.9x
+// and 1.6x it changed -1 to +7 %, so plans below 1.5x keep whole tiles.
+void build_launch(const uvt_plan_s& pl, int cus, std::vector<LaunchTile> launch[2],
+                  std::vector<PartSum>& sums, int64_t& nparts)
+{
+    for (int ph = 0; ph < 2; ++ph) launch[ph].clear();
+    sums.clear();
+    nparts = 0;
+    for (int ph = 0; ph < pl.phases; ++ph)
+        for (int64_t i = 0; i < pl.ntiles[ph]; ++i) launch[ph].push_back(LaunchTile{pl.tiles[ph][i], -1});
+    if (pl.phases != 1 || pl.flops < kSplitMinFlops || pl.ntiles[0] == 0) return;
+    const int64_t slots = (int64_t)cus * blocks_per_cu(pl.cfg);
+    int64_t work = 0, deepest = 0;  // k steps, summed over the tiles and the longest
+    for (int64_t i = 0; i < pl.ntiles[0]; ++i) {
+        const int64_t len = (int64_t)pl.tiles[0][i].k_hi - pl.tiles[0][i].k_lo;
+        work += len;
+        deepest = len > deepest ? len : deepest;
+    }
+    const int64_t share = work / slots;
+    if (2 * deepest < 3 * share) return;
+    int64_t kc = (share / 2 + 127) / 128 * 128;
+    kc = kc < 512 ? 512 : kc;
+    std::vector<LaunchTile> cut;
+    std::vector<PartSum> cut_sums;
+    int64_t cut_parts = 0;
+    for (int64_t i = 0; i < pl.ntiles[0]; ++i) {
+        const uvt_tile& t = pl.tiles[0][i];
+        const int64_t len = (int64_t)t.k_hi - t.k_lo;
+        const int64_t np = len > kc ? (len + kc - 1) / kc : 1;
+        if (np == 1) {
+            cut.push_back(LaunchTile{t, -1});
+            continue;
+        }
+        // piece boundaries on whole 16-deep panels, as even as that allows
+        cut_sums.push_back(PartSum{t.r0, t.rows, t.c0, (int32_t)cut_parts, (int32_t)(np - 1)});
+        for (int64_t q = 0; q < np; ++q) {
+            uvt_tile piece = t;
+            piece.k_lo = q == 0 ? t.k_lo : (int32_t)(t.k_lo + (len * q / np) / 16 * 16);
+            piece.k_hi = q == np - 1 ? t.k_hi : (int32_t)(t.k_lo + (len * (q + 1) / np) / 16 * 16);
+            cut.push_back(LaunchTile{piece, q == 0 ? -1 : (int32_t)cut_parts++});
+        }
+    }
+    const size_t tile_bytes = (size_t)uvt_cfg_tile_m(pl.cfg) * uvt_cfg_tile_n(pl.cfg) * sizeof(double);
+    if ((size_t)cut_parts * tile_bytes > kMaxPartBytes) return;
+    // deepest first (the planner's order among equals)
+    std::stable_sort(cut.begin(), cut.end(), [](const LaunchTile& a, const LaunchTile& b) {
+        return a.t.k_hi - a.t.k_lo > b.t.k_hi - b.t.k_lo;
+    });
+    launch[0].swap(cut);
+    sums.swap(cut_sums);
+    nparts = cut_parts;
+}
+
+}  // namespace
 
 static uvt_status plan_upload(uvt_handle h, uvt_plan plan)
 {
     if (plan->dev_id == h->device) return UVT_OK;
     uvt_plan_release_device(plan);
-    for (int ph = 0; ph < plan->phases; ++ph) {
-        size_t bytes = (size_t)plan->ntiles[ph] * sizeof(uvt_tile);
-        if (!bytes) continue;
-        if (hipMalloc(&plan->dev_tiles[ph], bytes) != hipSuccess) {
+    std::vector<LaunchTile> launch[2];
+    std::vector<PartSum> sums;
+    int64_t nparts = 0;
+    build_launch(*plan, h->cus, launch, sums, nparts);
+    auto upload = [&](void** dst, const void* src, size_t bytes) -> uvt_status {
+        if (!bytes) return UVT_OK;
+        if (hipMalloc(dst, bytes) != hipSuccess) {
             (void)hipGetLastError();
-            uvt_plan_release_device(plan);
             return UVT_ERR_NOMEM;
         }
-        UVT_HIP(hipMemcpy(plan->dev_tiles[ph], plan->tiles[ph], bytes, hipMemcpyHostToDevice));
+        if (src && hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipGetLastError();
+            return UVT_ERR_HIP;
+        }
+        return UVT_OK;
+    };
+    plan->dev_id = h->device;  // what is allocated below belongs to this device
+    uvt_status st = UVT_OK;
+    for (int ph = 0; ph < 2 && st == UVT_OK; ++ph) {
+        st = upload(&plan->dev_launch[ph], launch[ph].data(), launch[ph].size() * sizeof(LaunchTile));
+        plan->nlaunch[ph] = (int64_t)launch[ph].size();
     }
-    plan->dev_id = h->device;
-    return UVT_OK;
+    if (st == UVT_OK) st = upload(&plan->dev_sums, sums.data(), sums.size() * sizeof(PartSum));
+    plan->nsums = (int64_t)sums.size();
+    const int32_t tm = uvt_cfg_tile_m(plan->cfg), tn = uvt_cfg_tile_n(plan->cfg);
+    if (st == UVT_OK) st = upload(&plan->dev_parts, nullptr, (size_t)nparts * tm * tn * sizeof(double));
+    plan->nparts = nparts;
+    if (st != UVT_OK) uvt_plan_release_device(plan);
+    return st;
 }
 
 // ---------------------------------------------------------------------
@@ -238,33 +371,33 @@ static uvt_status plan_upload(uvt_handle h, uvt_plan plan)
 // ---------------------------------------------------------------------
 
 template <int TM, int TN>
-static void launch_tiles(const uvt_tile* dtiles, int64_t ntiles, bool vec, const double* A,
+static void launch_tiles(const LaunchTile* dtiles, int64_t ntiles, bool vec, const double* A,
                          int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc,
-                         int64_t m, hipStream_t s)
+                         int64_t m, double* parts, hipStream_t s)
 {
     dim3 grid((unsigned)ntiles), block(64 * TileWaves<TM, TN>::M * TileWaves<TM, TN>::N);
     if (vec)
         hipLaunchKernelGGL((k_trmm_tiles_f64<TM, TN, true>), grid, block, 0, s, dtiles, A, lda, B,
-                           ldb, C, ldc, m);
+                           ldb, C, ldc, m, parts);
     else
         hipLaunchKernelGGL((k_trmm_tiles_f64<TM, TN, false>), grid, block, 0, s, dtiles, A, lda, B,
-                           ldb, C, ldc, m);
+                           ldb, C, ldc, m, parts);
 }
 
-uvt_status uvt_launch_tiles_f64(int32_t cfg, const uvt_tile* dtiles, int64_t ntiles,
-                                const double* A, int64_t lda, const double* B, int64_t ldb,
-                                double* C, int64_t ldc, int64_t m, hipStream_t s)
+static uvt_status launch_tiles_f64(int32_t cfg, const LaunchTile* dtiles, int64_t ntiles,
+                                   const double* A, int64_t lda, const double* B, int64_t ldb,
+                                   double* C, int64_t ldc, int64_t m, double* parts, hipStream_t s)
 {
     if (ntiles <= 0) return UVT_OK;
     if (ntiles > 0x7FFFFFFF) return UVT_ERR_SIZE;
     const bool vec = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && lda % 2 == 0 &&
                      ldb % 2 == 0;
     if (cfg == UVT_CFG_64x64)
-        launch_tiles<64, 64>(dtiles, ntiles, vec, A, lda, B, ldb, C, ldc, m, s);
+        launch_tiles<64, 64>(dtiles, ntiles, vec, A, lda, B, ldb, C, ldc, m, parts, s);
     else if (cfg == UVT_CFG_128x32)
-        launch_tiles<128, 32>(dtiles, ntiles, vec, A, lda, B, ldb, C, ldc, m, s);
+        launch_tiles<128, 32>(dtiles, ntiles, vec, A, lda, B, ldb, C, ldc, m, parts, s);
     else
-        launch_tiles<128, 128>(dtiles, ntiles, vec, A, lda, B, ldb, C, ldc, m, s);
+        launch_tiles<128, 128>(dtiles, ntiles, vec, A, lda, B, ldb, C, ldc, m, parts, s);
     UVT_HIP(hipGetLastError());
     return UVT_OK;
 }
@@ -310,11 +443,32 @@ extern "C" uvt_status uvt_trmm_f64(uvt_handle h, uvt_plan plan, const double* dA
     uvt_status st = plan_upload(h, plan);
     if (st != UVT_OK) return st;
     hipStream_t s = (hipStream_t)stream;
-    for (int ph = 0; ph < plan->phases; ++ph) {
-        st = uvt_launch_tiles_f64(plan->cfg, (const uvt_tile*)plan->dev_tiles[ph], plan->ntiles[ph],
-                                  dA, lda, dB, ldb, dC, ldc, plan->m, s);
+    double* parts = (double*)plan->dev_parts;
+    for (int ph = 0; ph < 2; ++ph) {
+        st = launch_tiles_f64(plan->cfg, (const LaunchTile*)plan->dev_launch[ph], plan->nlaunch[ph],
+                              dA, lda, dB, ldb, dC, ldc, plan->m, parts, s);
         if (st != UVT_OK) return st;
     }
+    if (plan->nsums > 0) {
+        if (plan->nsums > 0x7FFFFFFF) return UVT_ERR_SIZE;
+        const int32_t tm = uvt_cfg_tile_m(plan->cfg), tn = uvt_cfg_tile_n(plan->cfg);
+        hipLaunchKernelGGL(k_add_parts, dim3((unsigned)plan->nsums), dim3(256), 0, s,
+                           (const PartSum*)plan->dev_sums, (int)tn, (int)(tm * tn),
+                           (const double*)parts, dC, ldc, plan->m);
+        UVT_HIP(hipGetLastError());
+    }
+    return UVT_OK;
+}
+
+extern "C" uvt_status uvt_plan_launch_info(uvt_handle h, uvt_plan plan, int64_t* workgroups,
+                                           int64_t* partial_tiles)
+{
+    if (!h || !plan) return UVT_ERR_ARG;
+    UVT_HIP(hipSetDevice(h->device));
+    const uvt_status st = plan_upload(h, plan);
+    if (st != UVT_OK) return st;
+    if (workgroups) *workgroups = plan->nlaunch[0] + plan->nlaunch[1];
+    if (partial_tiles) *partial_tiles = plan->nparts;
     return UVT_OK;
 }
 

@@ -84,14 +84,19 @@ __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int
                                          const UVT_GLOBAL double* __restrict__ A, int64_t lda,
                                          const UVT_GLOBAL double* __restrict__ B, int64_t ldb,
                                          UVT_GLOBAL double* __restrict__ C, int64_t ldc, int64_t m,
-                                         double* smem, int thread = -1)
+                                         double* smem, int thread = -1,
+                                         UVT_GLOBAL double* __restrict__ part = nullptr)
 {
     using G = F64Cfg<TM, TN, WAVES_M, WAVES_N, BK_>;
     constexpr int BK = G::BK, NT = G::NT;
     // thread: the caller's thread index when it must stay opaque to the
     // compiler (a persistent kernel running several tile shapes, whose
     // per-thread offsets would otherwise be hoisted out of its tile loop for
-    // every shape at once and spilled); -1: threadIdx.x
+    // every shape at once and spilled); -1: threadIdx.x.
+    // part: if set, the tile's product goes there instead of to C, as a
+    // TM x TN row-major block (row r, column c of the tile at r * TN + c;
+    // only rows < rows and columns inside C are written), to be added to C
+    // later in a fixed order.
     const int tid = thread >= 0 ? thread : (int)threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: branches on it stay scalar
@@ -275,9 +280,13 @@ __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int
             for (int v = 0; v < 4; ++v) {
                 const int row = wm * G::WM + i * 16 + (lane >> 4) + 4 * v;
                 if (cv && row < rows) {
-                    UVT_GLOBAL double* p = C + (int64_t)(r0 + row) * ldc + (c0 + col);
-                    if (accum) *p += acc[i][j][v];
-                    else *p = acc[i][j][v];
+                    if (part) {
+                        part[row * TN + col] = acc[i][j][v];
+                    } else {
+                        UVT_GLOBAL double* p = C + (int64_t)(r0 + row) * ldc + (c0 + col);
+                        if (accum) *p += acc[i][j][v];
+                        else *p = acc[i][j][v];
+                    }
                 }
             }
         }

@@ -33,9 +33,17 @@ struct uvt_plan_s {
     int64_t ntiles[2];
     uvt_tile *tiles[2];
     double flops, issued;
-    /* device copies, owned by the plan, created by the HIP side */
+    /* device state, owned by the plan, created by the HIP side on first use
+     * on a device: launch records of one or two launches, and, when tiles
+     * write partial products, the partial tiles and the records that add
+     * them to C */
     int dev_id;
-    void *dev_tiles[2];
+    void *dev_launch[2];
+    int64_t nlaunch[2];
+    void *dev_sums;
+    int64_t nsums;
+    void *dev_parts;
+    int64_t nparts;
 };
 
 /* Planner entry used by the batch path: plan one problem with a given tile

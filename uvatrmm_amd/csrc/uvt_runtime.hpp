@@ -37,9 +37,6 @@ UVT_INTERNAL uvt_status uvt_noalloc_setup(uvt_handle h);
     } while (0)
 
 UVT_INTERNAL uvt_status uvt_handle_workspace(uvt_handle h, size_t bytes, void** p);
-UVT_INTERNAL uvt_status uvt_launch_tiles_f64(int32_t cfg, const uvt_tile* dtiles, int64_t ntiles,
-                                const double* A, int64_t lda, const double* B, int64_t ldb,
-                                double* C, int64_t ldc, int64_t m, hipStream_t s);
 // FP64 C = tril(A) * B as one trapezoid walk with the given tile shape,
 // tile coordinates from the workgroup index (no plan, nothing to upload).
 UVT_INTERNAL uvt_status uvt_launch_trapezoid_f64(int32_t cfg, int64_t n, int64_t m, const double* A, int64_t lda,
