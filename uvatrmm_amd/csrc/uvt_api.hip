@@ -33,20 +33,28 @@ struct PartSum {
     int32_t r0, rows, c0, first, count;
 };
 
-template <int TM, int TN, bool VEC>
+// PARTS: the records are LaunchTiles (some write partial tiles); else plain
+// uvt_tiles, every one writing C.
+template <int TM, int TN, bool VEC, bool PARTS>
 __global__ __launch_bounds__((64 * TileWaves<TM, TN>::M * TileWaves<TM, TN>::N),
                              (TileWaves<TM, TN>::WAVES_PER_EU))
-void k_trmm_tiles_f64(const LaunchTile* __restrict__ tiles, const double* __restrict__ A, int64_t lda,
+void k_trmm_tiles_f64(const void* __restrict__ tiles, const double* __restrict__ A, int64_t lda,
                       const double* __restrict__ B, int64_t ldb, double* __restrict__ C,
                       int64_t ldc, int64_t m, double* __restrict__ parts)
 {
     using W = TileWaves<TM, TN>;
     __shared__ __attribute__((aligned(16))) double smem[F64Cfg<TM, TN, W::M, W::N>::SMEM_DOUBLES];
-    const LaunchTile lt = tiles[blockIdx.x];
-    const uvt_tile& t = lt.t;
-    UVT_GLOBAL double* part = lt.part >= 0 ? gbl(parts) + (int64_t)lt.part * (TM * TN) : nullptr;
-    f64_tile<TM, TN, VEC, W::M, W::N>(t.r0, t.rows, t.c0, t.k_lo, t.k_hi, t.flags, gbl(A), lda,
-                                      gbl(B), ldb, gbl(C), ldc, m, smem, -1, part);
+    if constexpr (PARTS) {
+        const LaunchTile lt = static_cast<const LaunchTile*>(tiles)[blockIdx.x];
+        const uvt_tile& t = lt.t;
+        UVT_GLOBAL double* part = lt.part >= 0 ? gbl(parts) + (int64_t)lt.part * (TM * TN) : nullptr;
+        f64_tile<TM, TN, VEC, W::M, W::N>(t.r0, t.rows, t.c0, t.k_lo, t.k_hi, t.flags, gbl(A), lda,
+                                          gbl(B), ldb, gbl(C), ldc, m, smem, -1, part);
+    } else {
+        const uvt_tile t = static_cast<const uvt_tile*>(tiles)[blockIdx.x];
+        f64_tile<TM, TN, VEC, W::M, W::N>(t.r0, t.rows, t.c0, t.k_lo, t.k_hi, t.flags, gbl(A), lda,
+                                          gbl(B), ldb, gbl(C), ldc, m, smem);
+    }
 }
 
 // Adds the partial tiles of each record to C in the record's order (one
@@ -276,8 +284,10 @@ int blocks_per_cu(int32_t cfg)
 // that share, kc, into pieces that run at once: the first piece writes C,
 // the others write partial tiles, which k_add_parts adds to C afterwards in
 // k order.  Measured at 9000 x 250 (the deepest tile 2.7x the share): 0.73
-// -> 0.45 ms; at 4096 x 4096 (0.97x) cutting cost 7 %, and between 0.9x
-// and 1.6x it changed -1 to +7 %, so plans below 1.5x keep whole tiles.
+// -> 0.45 ms; at 4096 x 4096 (0.97x) cutting cost 7 %, between 0.9x and
+// 1.6x it changed -1 to +7 %, and cutting the boxes of the N=2048 boxes
+// plan (1.5x) cost 11 %, so boxes plans and plans below 1.5x keep whole
+// tiles (and the kernel without partial tiles).
 void build_launch(const uvt_plan_s& pl, int cus, std::vector<LaunchTile> launch[2],
                   std::vector<PartSum>& sums, int64_t& nparts)
 {
@@ -354,7 +364,13 @@ static uvt_status plan_upload(uvt_handle h, uvt_plan plan)
     plan->dev_id = h->device;  // what is allocated below belongs to this device
     uvt_status st = UVT_OK;
     for (int ph = 0; ph < 2 && st == UVT_OK; ++ph) {
-        st = upload(&plan->dev_launch[ph], launch[ph].data(), launch[ph].size() * sizeof(LaunchTile));
+        if (nparts > 0) {
+            st = upload(&plan->dev_launch[ph], launch[ph].data(), launch[ph].size() * sizeof(LaunchTile));
+        } else {  // whole tiles: the plain records of the kernel without partial tiles
+            std::vector<uvt_tile> plain;
+            for (const LaunchTile& lt : launch[ph]) plain.push_back(lt.t);
+            st = upload(&plan->dev_launch[ph], plain.data(), plain.size() * sizeof(uvt_tile));
+        }
         plan->nlaunch[ph] = (int64_t)launch[ph].size();
     }
     if (st == UVT_OK) st = upload(&plan->dev_sums, sums.data(), sums.size() * sizeof(PartSum));
@@ -370,21 +386,31 @@ static uvt_status plan_upload(uvt_handle h, uvt_plan plan)
 // FP64 TRMM
 // ---------------------------------------------------------------------
 
+// parts == nullptr: dtiles holds uvt_tiles; else LaunchTiles.
 template <int TM, int TN>
-static void launch_tiles(const LaunchTile* dtiles, int64_t ntiles, bool vec, const double* A,
+static void launch_tiles(const void* dtiles, int64_t ntiles, bool vec, const double* A,
                          int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc,
                          int64_t m, double* parts, hipStream_t s)
 {
     dim3 grid((unsigned)ntiles), block(64 * TileWaves<TM, TN>::M * TileWaves<TM, TN>::N);
-    if (vec)
-        hipLaunchKernelGGL((k_trmm_tiles_f64<TM, TN, true>), grid, block, 0, s, dtiles, A, lda, B,
-                           ldb, C, ldc, m, parts);
-    else
-        hipLaunchKernelGGL((k_trmm_tiles_f64<TM, TN, false>), grid, block, 0, s, dtiles, A, lda, B,
-                           ldb, C, ldc, m, parts);
+    if (parts) {
+        if (vec)
+            hipLaunchKernelGGL((k_trmm_tiles_f64<TM, TN, true, true>), grid, block, 0, s, dtiles, A, lda,
+                               B, ldb, C, ldc, m, parts);
+        else
+            hipLaunchKernelGGL((k_trmm_tiles_f64<TM, TN, false, true>), grid, block, 0, s, dtiles, A, lda,
+                               B, ldb, C, ldc, m, parts);
+    } else {
+        if (vec)
+            hipLaunchKernelGGL((k_trmm_tiles_f64<TM, TN, true, false>), grid, block, 0, s, dtiles, A, lda,
+                               B, ldb, C, ldc, m, parts);
+        else
+            hipLaunchKernelGGL((k_trmm_tiles_f64<TM, TN, false, false>), grid, block, 0, s, dtiles, A, lda,
+                               B, ldb, C, ldc, m, parts);
+    }
 }
 
-static uvt_status launch_tiles_f64(int32_t cfg, const LaunchTile* dtiles, int64_t ntiles,
+static uvt_status launch_tiles_f64(int32_t cfg, const void* dtiles, int64_t ntiles,
                                    const double* A, int64_t lda, const double* B, int64_t ldb,
                                    double* C, int64_t ldc, int64_t m, double* parts, hipStream_t s)
 {
@@ -444,18 +470,18 @@ extern "C" uvt_status uvt_trmm_f64(uvt_handle h, uvt_plan plan, const double* dA
     if (st != UVT_OK) return st;
     hipStream_t s = (hipStream_t)stream;
     double* parts = (double*)plan->dev_parts;
+    if (plan->nsums > 0x7FFFFFFF) return UVT_ERR_SIZE;
     for (int ph = 0; ph < 2; ++ph) {
-        st = launch_tiles_f64(plan->cfg, (const LaunchTile*)plan->dev_launch[ph], plan->nlaunch[ph],
-                              dA, lda, dB, ldb, dC, ldc, plan->m, parts, s);
+        st = launch_tiles_f64(plan->cfg, plan->dev_launch[ph], plan->nlaunch[ph], dA, lda, dB, ldb,
+                              dC, ldc, plan->m, plan->nparts > 0 ? parts : nullptr, s);
         if (st != UVT_OK) return st;
-    }
-    if (plan->nsums > 0) {
-        if (plan->nsums > 0x7FFFFFFF) return UVT_ERR_SIZE;
-        const int32_t tm = uvt_cfg_tile_m(plan->cfg), tn = uvt_cfg_tile_n(plan->cfg);
-        hipLaunchKernelGGL(k_add_parts, dim3((unsigned)plan->nsums), dim3(256), 0, s,
-                           (const PartSum*)plan->dev_sums, (int)tn, (int)(tm * tn),
-                           (const double*)parts, dC, ldc, plan->m);
-        UVT_HIP(hipGetLastError());
+        if (ph == 0 && plan->nsums > 0) {  // the partial tiles belong to phase 0
+            const int32_t tm = uvt_cfg_tile_m(plan->cfg), tn = uvt_cfg_tile_n(plan->cfg);
+            hipLaunchKernelGGL(k_add_parts, dim3((unsigned)plan->nsums), dim3(256), 0, s,
+                               (const PartSum*)plan->dev_sums, (int)tn, (int)(tm * tn),
+                               (const double*)parts, dC, ldc, plan->m);
+            UVT_HIP(hipGetLastError());
+        }
     }
     return UVT_OK;
 }
