@@ -184,6 +184,21 @@ def test_error_statement_against_the_oracle(gpu, oracle, n, m, spread, slices):
     assert 0.25 <= err / est <= 4.0
 
 
+@pytest.mark.parametrize("n,m", [(1, 1), (5, 3), (257, 1), (259, 2), (300, 3), (520, 17)])
+def test_tiny_and_ragged_shapes(gpu, oracle, n, m):
+    """One row, one column, a box of one row (n = 257), B narrower than one
+    16-column fragment: against the oracle, with the stated bound covering
+    the error."""
+    A = oracle.seeded_A(n, 40 + n, 1)
+    B = oracle.seeded_B(n, m, 40 + m, 1)
+    say("tiny/ragged", (n, m))
+    C, bound, est = split_with_error(gpu, A, B, 13)
+    err = rel_fro(C, oracle.trmm(A, B))
+    say("err", err, "bound", bound, "estimate", est)
+    assert err <= TOL
+    assert err <= bound
+
+
 def test_extreme_row_scales(gpu, oracle):
     """Rows of A scaled by 2^s, s from -950 to 950 (their sums of squares
     leave the FP64 range): each row keeps the split path's relative accuracy
