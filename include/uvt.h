@@ -135,17 +135,22 @@ int32_t uvt_device_cus(uvt_handle h);
  * rows above k in k's 128-row tile through A's structural zeros (0 * Inf =
  * NaN), where a BLAS dtrmm would leave them finite; finite inputs are not
  * affected.  On its first use on a device the plan allocates its launch
- * records there, and, for a trapezoid plan of at least 2e9 flops whose
- * deepest tile is 1.5x or more an even share of the work per workgroup slot
- * (narrow B: few tiles), up to 64 MiB of partial tiles: its long reductions
- * run in pieces at once, and the pieces after the first are added to C in
- * k order (results repeat bit for bit).  Those belong to the plan: calls
- * with one plan on different streams must be serialized. */
+ * records there.  A trapezoid plan of at least 2e9 flops whose whole tiles
+ * would not finish together (narrow B: few, deep tiles) splits its long
+ * reductions and adds the pieces after the first of a tile to C afterwards
+ * in k order (results repeat bit for bit), with up to 64 MiB of partial
+ * tiles on the device: with 128x128 tiles whose whole-tile span is 1.3x or
+ * more an even share of the work, stream-K (one workgroup per slot, each the
+ * same number of k panels); with smaller tiles whose deepest tile is 1.5x or
+ * more a slot's share, cuts of those tiles into pieces.  The device state
+ * belongs to the plan: calls with one plan on different streams must be
+ * serialized. */
 uvt_status uvt_trmm_f64(uvt_handle h, uvt_plan plan, const double *dA, int64_t lda,
                         const double *dB, int64_t ldb, double *dC, int64_t ldc, void *stream);
 /* How the plan launches on the handle's device (placing its device copies
- * there if needed): workgroups of its tile launches, and partial tiles
- * (0 when every tile writes C directly).  Either pointer may be NULL. */
+ * there if needed): workgroups of its tile launches (for stream-K, one per
+ * workgroup slot), and partial tiles (0 when every tile writes C directly).
+ * Either pointer may be NULL. */
 uvt_status uvt_plan_launch_info(uvt_handle h, uvt_plan plan, int64_t *workgroups, int64_t *partial_tiles);
 
 /* Split-precision C ~= tril(A) * B: the box below each 256-row diagonal
@@ -278,7 +283,10 @@ uvt_status uvt_tune_choice(const char *tune_table, int64_t n, int64_t m, int32_t
 /* Serving (no counterpart in the reference).                          */
 /* ------------------------------------------------------------------ */
 /* A server owns a scheduler thread, a device memory arena of hbm_budget
- * bytes and max_slots streams.  Submitted requests wait in FIFO order and
+ * bytes and max_slots streams, each with a fixed buffer for the partial
+ * tiles of a stream-K FP64 product (64 MiB on a GPU of 256 CUs; an FP64
+ * request runs stream-K where a plan of its size would).  Submitted
+ * requests wait in FIFO order and
  * are admitted as soon as a slot is free and their operands (A, B, C and,
  * for split requests, the workspace) fit in the unused part of the arena;
  * admitted requests run concurrently on their slot streams, so requests that
@@ -292,7 +300,7 @@ uvt_status uvt_tune_choice(const char *tune_table, int64_t n, int64_t m, int32_t
  * Sharing the GPU among fewer large requests shortens each of them, which
  * is what sets the tail latency of a mixed stream. */
 typedef struct {
-    size_t hbm_budget; /* bytes of device memory the server may hold */
+    size_t hbm_budget; /* bytes of the operand arena (the slots' partial tiles come on top) */
     int32_t max_slots; /* requests in flight on the device (1..64) */
     int32_t slices;    /* moduli for split-class requests */
     const char *tune_table; /* FP64 tile shapes (NULL: built-in) */

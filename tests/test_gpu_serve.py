@@ -53,8 +53,10 @@ def serve(uvt, gpu, reqs, budget, slots):
 
 
 def test_serve_alone_equals_mix(uvt, gpu, oracle):
-    # the last two are split-class requests large enough for the int8 path
-    reqs = mixed_requests(40, seed=11) + [(4096, 1024, 1, 6001), (6144, 640, 1, 6002)]
+    # two FP64 requests with few, deep 128x128 tiles (their products run
+    # stream-K), then two split-class requests large enough for the int8 path
+    sk = [(8165, 939, 0, 6003), (5000, 2000, 0, 6004)]
+    reqs = mixed_requests(40, seed=11) + sk + [(4096, 1024, 1, 6001), (6144, 640, 1, 6002)]
     say("requests", len(reqs), "alone")
     alone, st1, _ = serve(uvt, gpu, reqs, 8 * GiB, 1)
     assert st1["peak_inflight"] == 1
@@ -69,6 +71,13 @@ def test_serve_alone_equals_mix(uvt, gpu, oracle):
         ref = oracle.trmm(oracle.seeded_A(n, seed, 1), oracle.seeded_B(n, m, seed, 1))
         err = rel_fro(mix[k], ref)
         assert err <= (1e-13 if prec == 0 else 1e-12), (k, n, m, prec, err)
+    for k in range(40, 42):  # the stream-K requests: exact (double-double) rows
+        n, m, prec, seed = reqs[k]
+        rows = np.unique(np.concatenate([[0, 127, 128, n - 1], np.arange(0, n, 400)]))
+        A, B = oracle.seeded_A(n, seed, 1), oracle.seeded_B(n, m, seed, 1)
+        err = rel_fro(mix[k][rows], oracle.trmm_rows_dd(A, B, rows))
+        say("stream-k request", (n, m), "err", err)
+        assert err <= 1e-13
     # the two large split-class requests ran on the int8 path and state their
     # error: the bound covers the error against the oracle; every other
     # request ran in FP64 and states none

@@ -217,32 +217,55 @@ def test_boxes_equals_trapezoid_and_repeatable(gpu, uvt, oracle):
     np.testing.assert_array_equal(c2, c3)
 
 
-@pytest.mark.parametrize("n,m,p,mapping,shape,cut", [
-    (9000, 250, 1, "balanced", "trapezoid", True),     # narrow B: the deepest tile 2.7x a slot's share
-    (12000, 128, 4, "reg", "trapezoid", True),         # 4x
-    (2048, 2048, 4, "reg", "trapezoid", False),        # 1.45x: whole tiles
-    (4096, 4096, 1, "reg", "trapezoid", False),        # 0.97x
-    (2048, 2048, 4, "reg", "boxes", False),            # the reference's N=2048 configuration
+@pytest.mark.parametrize("n,m,p,mapping,shape,pad,how", [
+    # 128x128 tiles, few and deep (narrow B): stream-K
+    (8165, 939, 16, "balanced", "trapezoid", 0, "sk"),  # 512 tiles, depths 128..8165
+    (5000, 2000, 1, "reg", "trapezoid", 1, "sk"),       # odd lda (8-byte loads), an 8-row last row tile
+    (6772, 1548, 4, "reg", "trapezoid", 0, "sk"),
+    # 64x64 tiles, few and deep: whole tiles cut into k pieces
+    (9000, 250, 1, "balanced", "trapezoid", 0, "cut"),
+    (12000, 128, 4, "reg", "trapezoid", 0, "cut"),
+    # whole tiles
+    (4096, 4096, 1, "reg", "trapezoid", 0, "whole"),    # 128x128 tiles that balance by themselves
+    (2048, 2048, 4, "reg", "boxes", 0, "whole"),        # the reference's N=2048 configuration
+    (1030, 37, 1, "reg", "trapezoid", 0, "whole"),      # under 2e9 flops
 ])
-def test_partial_tiles(gpu, uvt, oracle, n, m, p, mapping, shape, cut):
-    """Plans whose long reductions run in pieces, the pieces after the first
-    added to C afterwards in k order (narrow B, few tiles), and plans that
-    keep whole tiles: against exact double-double rows, and the same bits on
+def test_split_reductions(gpu, uvt, oracle, n, m, p, mapping, shape, pad, how):
+    """Trapezoid plans whose whole tiles would not finish together split
+    their reductions: with 128x128 tiles stream-K (one workgroup per slot,
+    each the same number of k panels), with smaller tiles the deepest tiles
+    cut into k pieces; the pieces after the first of a tile are added to C
+    afterwards in k order.  Small-integer inputs give exact sums in any
+    order, so they show every k panel of every tile counted exactly once;
+    seeded inputs against exact double-double rows, and the same bits on
     every run."""
     plan = uvt.Plan(n, m, p, mapping, shape)
     wg, parts = gpu.plan_launch_info(plan)
     plan.close()
     say("launch", (n, m, p, mapping, shape), "workgroups", wg, "partial tiles", parts)
-    assert (parts > 0) == cut
+    slots = 2 * gpu.cus
+    if how == "sk":
+        assert wg == slots and parts > 0
+    elif how == "cut":
+        assert wg > slots and parts > 0
+    else:
+        assert parts == 0
+    lda = n + pad
+    rng = np.random.default_rng(n + m)
+    Ai = np.tril(rng.integers(-3, 4, (n, n)).astype(np.float64))
+    Bi = rng.integers(-3, 4, (n, m)).astype(np.float64)
+    Ap = np.zeros((n, lda))
+    Ap[:, :n] = Ai
+    np.testing.assert_array_equal(run_trmm(gpu, uvt, Ap, Bi, p, mapping, shape, lda=lda), Ai @ Bi)
+    del Ai, Bi, Ap
     A, B = oracle.seeded_A(n, 31, 1), oracle.seeded_B(n, m, 31, 1)
     c1 = run_trmm(gpu, uvt, A, B, p, mapping, shape)
     c2 = run_trmm(gpu, uvt, A, B, p, mapping, shape)
     np.testing.assert_array_equal(c1, c2)
-    rng = np.random.default_rng(n + m)
     rows = np.unique(np.concatenate([[0, 1, 63, 64, 127, 128, n // 2, n - 2, n - 1],
                                      np.arange(0, n, 512), rng.integers(0, n, 24)]))
     err = rel_fro(c1[rows], oracle.trmm_rows_dd(A, B, rows))
-    say("partial tiles", (n, m, p, mapping, shape), "err", err)
+    say("split reductions", how, (n, m, p, mapping, shape), "err", err)
     assert err <= 1e-13
     if shape == "boxes":
         c3 = run_trmm(gpu, uvt, A, B, p, mapping, "trapezoid")

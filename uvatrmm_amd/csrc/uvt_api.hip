@@ -11,6 +11,7 @@
 #include "uvt_f64_tile.hpp"
 #include "uvt_internal.h"
 #include "uvt_runtime.hpp"
+#include "uvt_sk.hpp"
 
 using namespace uvt;
 
@@ -92,6 +93,85 @@ void k_trmm_trapezoid_f64(int64_t n, int64_t m, int nrt, int nct, const double* 
     const int rows = (int)(n - r0 < TM ? n - r0 : TM);
     f64_tile<TM, TN, VEC, W::M, W::N>(r0, rows, ct * TN, 0, r0 + rows, kTileLower, gbl(A), lda,
                                       gbl(B), ldb, gbl(C), ldc, m, smem);
+}
+
+// The trapezoid walk split stream-K (uvt_sk.hpp): workgroup b runs the
+// pieces of panels [b P / G, (b+1) P / G) one after another; its first piece
+// writes partial tile b if it starts inside its tile, every other piece
+// starts at k = 0 and writes C.
+template <int TM, int TN, bool VEC>
+__global__ __launch_bounds__((64 * TileWaves<TM, TN>::M * TileWaves<TM, TN>::N),
+                             (TileWaves<TM, TN>::WAVES_PER_EU))
+void k_trmm_sk_f64(SkGeo g, const double* __restrict__ A, int64_t lda, const double* __restrict__ B,
+                   int64_t ldb, double* __restrict__ C, int64_t ldc, double* __restrict__ parts)
+{
+    using W = TileWaves<TM, TN>;
+    static_assert(F64Cfg<TM, TN, W::M, W::N>::BK == kSkPanel, "stream-K panels are the kernel's k panels");
+    __shared__ __attribute__((aligned(16))) double smem[F64Cfg<TM, TN, W::M, W::N>::SMEM_DOUBLES];
+    // The piece arithmetic (64-bit divisions) runs in vector registers; its
+    // results are the same in every lane and go back to scalar registers, so
+    // they do not take vector registers from the tile.
+    auto uni = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
+    auto uni64 = [&](int64_t v) {
+        return (int64_t)(((uint64_t)(uint32_t)uni((int)(v >> 32)) << 32) | (uint32_t)uni((int)v));
+    };
+    // Workgroup i runs on XCD i % 8: the 8 XCDs take contiguous eighths of
+    // the sequence, so the workgroups that share an XCD's L2 work on
+    // neighbouring tiles (the same rows of A at about the same k).
+    const int64_t i = blockIdx.x;
+    const int64_t b = g.G % kXcds == 0 ? (i % kXcds) * (g.G / kXcds) + i / kXcds : i;
+    int64_t x = uni64(sk_start(g, b));
+    const int64_t xe = uni64(sk_start(g, b + 1));
+    UVT_GLOBAL double* part = gbl(parts) + b * (TM * TN);
+    bool first = true;
+    while (x < xe) {
+        const SkPiece s = sk_piece(g, x, xe);
+        if (!first) __syncthreads();  // the LDS tile is reused by the next piece
+        // the thread index, opaque per piece: the tile's per-thread offsets are
+        // then computed inside each piece instead of held (and spilled) across
+        // the piece loop
+        int tid = (int)threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        f64_tile<TM, TN, VEC, W::M, W::N>(uni(s.r0), uni(s.rows), uni(s.c0), uni(s.k_lo), uni(s.k_hi),
+                                          kTileLower, gbl(A), lda, gbl(B), ldb, gbl(C), ldc, g.m, smem,
+                                          tid, uni(s.p0 > 0) ? part : nullptr);
+        first = false;
+        x = uni64(s.next);
+    }
+}
+
+// After k_trmm_sk_f64: for each tile split between workgroups, the
+// workgroup b whose first piece is the tile's first partial tile adds
+// partial tiles b, b+1, ... (k order) to C, which holds the k = 0 piece.
+// (gridDim.y workgroups per split tile, each over its share of the rows.)
+template <int TM, int TN>
+__global__ __launch_bounds__(256) void k_sk_fixup(SkGeo g, const double* __restrict__ parts,
+                                                  double* __restrict__ C, int64_t ldc)
+{
+    const int64_t b = blockIdx.x;
+    if (b == 0) return;
+    const int64_t x = sk_start(g, b);
+    const int j = sk_row(g, x);
+    const int64_t d = sk_depth(g, j), off = x - sk_before(g, j), p0 = off % d;
+    if (p0 == 0) return;                     // b starts its tile: it wrote C
+    const int64_t tile0 = x - p0;
+    if (sk_start(g, b - 1) > tile0) return;  // b - 1 holds this tile's first partial tile
+    int64_t cnt = 1;
+    while (b + cnt < g.G && sk_start(g, b + cnt) < tile0 + d) ++cnt;
+    const int r0 = j * TM, rows = (int)(g.n - r0 < TM ? g.n - r0 : TM);
+    const int64_t c0 = (off / d) * TN;
+    const UVT_GLOBAL double* P = gbl(parts) + b * (TM * TN);
+    const int per = (rows + (int)gridDim.y - 1) / (int)gridDim.y;
+    const int e_lo = (int)blockIdx.y * per * TN;
+    const int e_hi = min(rows, ((int)blockIdx.y + 1) * per) * TN;
+    for (int e = e_lo + (int)threadIdx.x; e < e_hi; e += 256) {
+        const int r = e / TN, c = e % TN;
+        if (c0 + c >= g.m) continue;
+        UVT_GLOBAL double* p = gbl(C) + (int64_t)(r0 + r) * ldc + (c0 + c);
+        double v = *p;
+        for (int64_t q = 0; q < cnt; ++q) v += P[q * (TM * TN) + e];
+        *p = v;
+    }
 }
 
 // One wave per row: sum_j |C[i][j]|, lanes strided over columns, then a
@@ -258,7 +338,7 @@ extern "C" void uvt_plan_release_device(uvt_plan plan)
             *b = nullptr;
         }
     }
-    plan->nlaunch[0] = plan->nlaunch[1] = plan->nsums = plan->nparts = 0;
+    plan->nlaunch[0] = plan->nlaunch[1] = plan->nsums = plan->nparts = plan->sk_grid = 0;
     plan->dev_id = -1;
 }
 
@@ -345,6 +425,22 @@ static uvt_status plan_upload(uvt_handle h, uvt_plan plan)
 {
     if (plan->dev_id == h->device) return UVT_OK;
     uvt_plan_release_device(plan);
+    // A trapezoid plan whose whole tiles would not finish together runs
+    // stream-K: nothing to upload but its partial tiles.
+    const int64_t sk = plan->phases == 1 ? uvt_sk_workgroups(plan->cfg, plan->n, plan->m, h->cus) : 0;
+    if (sk > 0) {
+        if (hipMalloc(&plan->dev_parts, uvt_sk_parts_bytes(plan->cfg, sk)) != hipSuccess) {
+            (void)hipGetLastError();
+            plan->dev_parts = nullptr;
+            return UVT_ERR_NOMEM;
+        }
+        plan->dev_id = h->device;
+        plan->sk_grid = sk;
+        const SkGeo g = sk_geometry(plan->n, plan->m, uvt_cfg_tile_m(plan->cfg), uvt_cfg_tile_n(plan->cfg), sk);
+        for (int64_t b = 1; b < g.G; ++b)
+            if (sk_piece(g, sk_start(g, b), sk_start(g, b + 1)).p0 > 0) ++plan->nparts;
+        return UVT_OK;
+    }
     std::vector<LaunchTile> launch[2];
     std::vector<PartSum> sums;
     int64_t nparts = 0;
@@ -442,6 +538,94 @@ static void launch_trapezoid(bool vec, int64_t n, int64_t m, const double* A, in
                            A, lda, B, ldb, C, ldc);
 }
 
+template <int TM, int TN>
+static void launch_trapezoid_sk(bool vec, const SkGeo& g, const double* A, int64_t lda, const double* B,
+                                int64_t ldb, double* C, int64_t ldc, double* parts, hipStream_t s)
+{
+    const dim3 grid((unsigned)g.G), block(64 * TileWaves<TM, TN>::M * TileWaves<TM, TN>::N);
+    if (vec)
+        hipLaunchKernelGGL((k_trmm_sk_f64<TM, TN, true>), grid, block, 0, s, g, A, lda, B, ldb, C, ldc,
+                           parts);
+    else
+        hipLaunchKernelGGL((k_trmm_sk_f64<TM, TN, false>), grid, block, 0, s, g, A, lda, B, ldb, C, ldc,
+                           parts);
+    hipLaunchKernelGGL((k_sk_fixup<TM, TN>), dim3((unsigned)g.G, 4), dim3(256), 0, s, g, parts, C, ldc);
+}
+
+namespace {
+
+// When the trapezoid walk runs stream-K (measured on MI355X, 26 shapes from
+// 1500 x 1500 to 16384 x 4096 against whole tiles): with 128x128 tiles whose
+// whole-tile span is 1.4-2.0x an even share (narrow B: 512-700 deep tiles on
+// 512 slots), stream-K took 0.73-0.97x the time (3000 x 3500 at 1.42x: 1.0x);
+// where whole tiles balance (span 1.0-1.1x) it took 1.05-1.23x, because its
+// workgroups no longer go through k together and share less in L2; with 64x64
+// tiles it took 1.02-1.25x at any span (there the partial-tile cut of the
+// plan path is faster).  More workgroups than slots (2x, 4x) did not help.
+constexpr double kSkMinFlops = 2e9;  // below: launch time, not balance, bounds a walk
+constexpr double kSkGain = 1.3;      // whole-tile span over an even share that stream-K needs
+
+// How long whole tiles take, in panels per workgroup slot: the launch hands
+// tiles, deepest first, to the slot that frees first (list scheduling).
+int64_t whole_tile_panels(const SkGeo& g, int64_t slots)
+{
+    std::vector<int64_t> heap(slots, 0);  // min-heap of the slots' finish times
+    auto later = [](int64_t a, int64_t b) { return a > b; };
+    int64_t span = 0;
+    for (int j = g.nrt - 1; j >= 0; --j) {
+        const int64_t d = sk_depth(g, j);
+        for (int c = 0; c < g.nct; ++c) {
+            std::pop_heap(heap.begin(), heap.end(), later);
+            heap.back() += d;
+            span = std::max(span, heap.back());
+            std::push_heap(heap.begin(), heap.end(), later);
+        }
+    }
+    return span;
+}
+
+}  // namespace
+
+int64_t uvt_sk_workgroups(int32_t cfg, int64_t n, int64_t m, int cus)
+{
+    if (cfg != UVT_CFG_128x128 || (double)m * (double)n * (double)n < kSkMinFlops || n > 0x3FFFFFFF ||
+        m > 0x3FFFFFFF)
+        return 0;
+    const int tm = uvt_cfg_tile_m(cfg), tn = uvt_cfg_tile_n(cfg);
+    const int64_t slots = (int64_t)cus * blocks_per_cu(cfg);
+    const SkGeo g = sk_geometry(n, m, tm, tn, slots);
+    const int64_t tiles = (int64_t)g.nrt * g.nct;
+    if (tiles > 16 * slots || g.G < slots) return 0;  // many tiles balance by themselves
+    const int64_t even = (g.P + slots - 1) / slots;
+    return (double)whole_tile_panels(g, slots) >= kSkGain * (double)even ? slots : 0;
+}
+
+size_t uvt_sk_parts_bytes(int32_t cfg, int64_t workgroups)
+{
+    return (size_t)workgroups * uvt_cfg_tile_m(cfg) * uvt_cfg_tile_n(cfg) * sizeof(double);
+}
+
+size_t uvt_sk_max_parts_bytes(int cus)
+{
+    return uvt_sk_parts_bytes(UVT_CFG_128x128, (int64_t)cus * blocks_per_cu(UVT_CFG_128x128));
+}
+
+uvt_status uvt_launch_trapezoid_sk_f64(int32_t cfg, int64_t n, int64_t m, const double* A, int64_t lda,
+                                       const double* B, int64_t ldb, double* C, int64_t ldc,
+                                       int64_t workgroups, double* parts, hipStream_t s)
+{
+    if (n <= 0 || m <= 0 || lda < n || ldb < m || ldc < m || workgroups <= 0 || !parts) return UVT_ERR_ARG;
+    if (n > 0x3FFFFFFF || m > 0x3FFFFFFF || workgroups > 0x7FFFFFFF) return UVT_ERR_SIZE;
+    const SkGeo g = sk_geometry(n, m, uvt_cfg_tile_m(cfg), uvt_cfg_tile_n(cfg), workgroups);
+    const bool vec = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && lda % 2 == 0 &&
+                     ldb % 2 == 0;
+    if (cfg == UVT_CFG_64x64) launch_trapezoid_sk<64, 64>(vec, g, A, lda, B, ldb, C, ldc, parts, s);
+    else if (cfg == UVT_CFG_128x32) launch_trapezoid_sk<128, 32>(vec, g, A, lda, B, ldb, C, ldc, parts, s);
+    else launch_trapezoid_sk<128, 128>(vec, g, A, lda, B, ldb, C, ldc, parts, s);
+    UVT_HIP(hipGetLastError());
+    return UVT_OK;
+}
+
 uvt_status uvt_launch_trapezoid_f64(int32_t cfg, int64_t n, int64_t m, const double* A, int64_t lda,
                                     const double* B, int64_t ldb, double* C, int64_t ldc,
                                     hipStream_t s)
@@ -470,6 +654,9 @@ extern "C" uvt_status uvt_trmm_f64(uvt_handle h, uvt_plan plan, const double* dA
     if (st != UVT_OK) return st;
     hipStream_t s = (hipStream_t)stream;
     double* parts = (double*)plan->dev_parts;
+    if (plan->sk_grid > 0)
+        return uvt_launch_trapezoid_sk_f64(plan->cfg, plan->n, plan->m, dA, lda, dB, ldb, dC, ldc,
+                                           plan->sk_grid, parts, s);
     if (plan->nsums > 0x7FFFFFFF) return UVT_ERR_SIZE;
     for (int ph = 0; ph < 2; ++ph) {
         st = launch_tiles_f64(plan->cfg, plan->dev_launch[ph], plan->nlaunch[ph], dA, lda, dB, ldb,
@@ -493,7 +680,7 @@ extern "C" uvt_status uvt_plan_launch_info(uvt_handle h, uvt_plan plan, int64_t*
     UVT_HIP(hipSetDevice(h->device));
     const uvt_status st = plan_upload(h, plan);
     if (st != UVT_OK) return st;
-    if (workgroups) *workgroups = plan->nlaunch[0] + plan->nlaunch[1];
+    if (workgroups) *workgroups = plan->sk_grid > 0 ? plan->sk_grid : plan->nlaunch[0] + plan->nlaunch[1];
     if (partial_tiles) *partial_tiles = plan->nparts;
     return UVT_OK;
 }

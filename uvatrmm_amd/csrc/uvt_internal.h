@@ -44,6 +44,7 @@ struct uvt_plan_s {
     int64_t nsums;
     void *dev_parts;
     int64_t nparts;
+    int64_t sk_grid; /* > 0: a stream-K launch of that many workgroups */
 };
 
 /* Planner entry used by the batch path: plan one problem with a given tile

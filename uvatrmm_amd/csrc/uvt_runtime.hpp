@@ -42,6 +42,21 @@ UVT_INTERNAL uvt_status uvt_handle_workspace(uvt_handle h, size_t bytes, void** 
 UVT_INTERNAL uvt_status uvt_launch_trapezoid_f64(int32_t cfg, int64_t n, int64_t m, const double* A, int64_t lda,
                                     const double* B, int64_t ldb, double* C, int64_t ldc,
                                     hipStream_t s);
+// The same walk split stream-K (uvt_sk.hpp) over `workgroups` workgroups, then
+// its fixup launch; parts holds uvt_sk_parts_bytes(cfg, workgroups) bytes.
+// Each result is the same on every run with the same arguments and GPU.
+UVT_INTERNAL uvt_status uvt_launch_trapezoid_sk_f64(int32_t cfg, int64_t n, int64_t m, const double* A,
+                                                    int64_t lda, const double* B, int64_t ldb, double* C,
+                                                    int64_t ldc, int64_t workgroups, double* parts,
+                                                    hipStream_t s);
+// Workgroups for the stream-K walk of an n x n by n x m product in cfg tiles
+// on a GPU of `cus` compute units (one per workgroup slot), or 0 for whole
+// tiles: stream-K runs 128x128 tiles of at least 2e9 flops whose whole-tile
+// span is at least 1.3x an even share of the work (narrow B: few, deep tiles).
+UVT_INTERNAL int64_t uvt_sk_workgroups(int32_t cfg, int64_t n, int64_t m, int cus);
+UVT_INTERNAL size_t uvt_sk_parts_bytes(int32_t cfg, int64_t workgroups);
+// The most partial-tile bytes any uvt_sk_workgroups result needs on `cus` CUs.
+UVT_INTERNAL size_t uvt_sk_max_parts_bytes(int cus);
 // FP64 C[0:M, 0:N] = A[0:M, 0:K] * B[0:K, 0:N] (no triangle), tile coordinates
 // from the workgroup index.
 UVT_INTERNAL uvt_status uvt_launch_gemm_f64(int32_t cfg, int64_t M, int64_t N, int64_t K, const double* A,

@@ -12,7 +12,9 @@
 //         (inputs, product, optional copy-back) on the slot's stream -> sleep
 //         until a submission or ~50 us.
 // A request's kernels depend only on its own (n, m, class), so its result is
-// the same alone and in the mix.
+// the same alone and in the mix.  Besides the arena, each slot holds a fixed
+// buffer for the partial tiles of a stream-K FP64 product (64 MiB on a GPU of
+// 256 CUs), reused by the slot's next request once the last one is done.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -86,6 +88,7 @@ struct Slot {
     size_t off = SIZE_MAX;  // the request's arena block (p.bytes long)
     uvt_status status = UVT_OK;
     const char* err = nullptr;  // int8 path: the error statement in the workspace
+    double* parts = nullptr;    // partial tiles of a stream-K FP64 product (the slot's own)
 };
 
 
@@ -193,7 +196,9 @@ static uvt_status launch(uvt_server s, Slot& sl)
         e = uvt_trmm_split(s->h, n, m, s->cfg.slices, dA, n, dB, m, dC, m, work, wbytes, st);
     } else {
         const int tile = s->tune.empty() ? uvt_default_cfg(n, m) : uvt_tune_pick(s->tune, n, m);
-        e = uvt_launch_trapezoid_f64(tile, n, m, dA, n, dB, m, dC, m, st);
+        const int64_t sk = uvt_sk_workgroups(tile, n, m, s->h->cus);
+        e = sk > 0 ? uvt_launch_trapezoid_sk_f64(tile, n, m, dA, n, dB, m, dC, m, sk, sl.parts, st)
+                   : uvt_launch_trapezoid_f64(tile, n, m, dA, n, dB, m, dC, m, st);
     }
     if (e != UVT_OK) return e;
     UVT_HIP(hipEventRecord(sl.done, st));
@@ -359,13 +364,16 @@ extern "C" uvt_status uvt_serve_create(uvt_handle h, const uvt_serve_config* cfg
         return UVT_ERR_NOMEM;
     }
     s->slots.resize(cfg->max_slots);
+    const size_t parts_bytes = uvt_sk_max_parts_bytes(h->cus);
     for (Slot& sl : s->slots) {
         if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) {
+            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess ||
+            hipMalloc((void**)&sl.parts, parts_bytes) != hipSuccess) {
             (void)hipGetLastError();
             for (Slot& x : s->slots) {
                 if (x.stream) (void)hipStreamDestroy(x.stream);
                 if (x.done) (void)hipEventDestroy(x.done);
+                if (x.parts) (void)hipFree(x.parts);
             }
             s->arena.release();
             delete s;
@@ -466,6 +474,7 @@ extern "C" uvt_status uvt_serve_destroy(uvt_server s)
     for (Slot& sl : s->slots) {
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
         if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.parts) (void)hipFree(sl.parts);
     }
     s->arena.release();
     delete s;
