@@ -606,7 +606,21 @@ def main():
         except Exception as e:  # the baseline is reported, never the measured path
             log("cpu baseline failed:", repr(e))
 
-    kernel = "k_trmm_tiles_f64<128,128,true> (8 waves of 32x64)"
+    # the kernels this plan runs on this GPU (whole tiles, stream-K, or cut reductions)
+    wg, parts = h.plan_launch_info(plan)
+    cfg0 = int(plan.tiles(0)["cfg"][0]) if len(plan.tiles(0)) else 0
+    tile, waves = {0: ("128,128", "8 waves of 32x64"), 1: ("64,64", "4 waves of 32x32"),
+                   2: ("128,32", "4 waves of 32x32")}[cfg0]
+    vec = "true" if n % 2 == 0 and m % 2 == 0 else "false"
+    launches = plan.phases
+    if parts and cfg0 == 0 and plan.phases == 1:
+        kernel = f"k_trmm_sk_f64<128,128,{vec}> (stream-K, {wg} workgroups) + k_sk_fixup"
+        launches = 2
+    elif parts:
+        kernel = f"k_trmm_tiles_f64<{tile},{vec}> ({waves}; reductions cut into pieces) + k_add_parts"
+        launches = plan.phases + 1
+    else:
+        kernel = f"k_trmm_tiles_f64<{tile},{vec}> ({waves})"
     out = {
         "metric": "TRMM TFLOP/s + % MFMA peak at N=32768 (plain FP64 = value; split in 'split')",
         "value": round(value, 4),
@@ -627,7 +641,7 @@ def main():
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                      "traffic": MEASURED_TRAFFIC.get((n, m, args.bands, args.map, args.shape)),
-                     "launches_per_step": plan.phases,
+                     "launches_per_step": launches,
                      "algorithmic_flops_per_step": flops},
         "cpu_baseline": cpu,
         "split": split,
