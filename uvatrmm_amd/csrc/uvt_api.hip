@@ -132,7 +132,7 @@ void k_trmm_sk_f64(SkGeo g, const double* __restrict__ A, int64_t lda, const dou
         // the piece loop
         int tid = (int)threadIdx.x;
         asm volatile("" : "+v"(tid));
-        f64_tile<TM, TN, VEC, W::M, W::N>(uni(s.r0), uni(s.rows), uni(s.c0), uni(s.k_lo), uni(s.k_hi),
+        f64_tile<TM, TN, VEC, W::M, W::N, kSkPanel, false>(uni(s.r0), uni(s.rows), uni(s.c0), uni(s.k_lo), uni(s.k_hi),
                                           kTileLower, gbl(A), lda, gbl(B), ldb, gbl(C), ldc, g.m, smem,
                                           tid, uni(s.p0 > 0) ? part : nullptr);
         first = false;

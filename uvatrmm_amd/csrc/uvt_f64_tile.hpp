@@ -79,7 +79,10 @@ struct F64Cfg {
 };
 
 // VEC: A, B 16-byte aligned and lda, ldb even, so 2-double loads are legal.
-template <int TM, int TN, bool VEC, int WAVES_M = 2, int WAVES_N = 2, int BK_ = 16>
+// EDGE_IDLE: waves of an edge tile that hold no row or column of C run no
+// products (off where a caller's own registers leave no room for the test:
+// the stream-K kernel then spills inside its masked k loop).
+template <int TM, int TN, bool VEC, int WAVES_M = 2, int WAVES_N = 2, int BK_ = 16, bool EDGE_IDLE = true>
 __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int k_hi, int flags,
                                          const UVT_GLOBAL double* __restrict__ A, int64_t lda,
                                          const UVT_GLOBAL double* __restrict__ B, int64_t ldb,
@@ -246,9 +249,14 @@ __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int
         // leaves the CU's matrix units to the other waves.  The panel count
         // is uniform over the wave.  Skipping adds of exact zeros changes no
         // result (the sign of a zero aside).
+        // Likewise a wave whose rows all lie past the tile's last row, or
+        // whose columns all lie past C's last column (a tile at the bottom or
+        // right edge, which never takes the steady-state loop), multiplies
+        // only zeros and writes nothing: it runs no products.
+        const bool idle = EDGE_IDLE && (wm * G::WM >= rows || (int64_t)c0 + wn * G::WN >= m);
         int nk_w = nk;
         if (lower) {
-            const int last = r0 + wm * G::WM + G::WM - 1;  // last row of this wave
+            const int last = idle ? -1 : r0 + wm * G::WM + G::WM - 1;  // last row of this wave
             nk_w = last >= k_start ? (last - k_start) / BK + 1 : 0;
             nk_w = nk_w < kt ? kt : (nk_w > nk ? nk : nk_w);
         }
