@@ -386,7 +386,11 @@ uvt_status uvt_fill_seeded(uvt_handle h, int64_t n, int64_t m, uint64_t seed, in
 /* ------------------------------------------------------------------ */
 uvt_status uvt_malloc(uvt_handle h, size_t bytes, void **dptr);
 uvt_status uvt_free(uvt_handle h, void *dptr);
-/* kind: 0 host->device, 1 device->host, 2 device->device; synchronous. */
+/* kind: 0 host->device, 1 device->host, 2 device->device; synchronous.
+ * Host copies of 1 MiB or more, and strided host copies, go through two
+ * 16 MiB pinned buffers that the handle allocates on first use and keeps
+ * (rows packed on the host, a chunk in flight while the next is packed):
+ * host memory is read or written as plain memory, never registered. */
 uvt_status uvt_memcpy(uvt_handle h, void *dst, const void *src, size_t bytes, int32_t kind);
 /* Strided copy of `height` rows of `width` bytes (pitches in bytes). */
 uvt_status uvt_memcpy_2d(uvt_handle h, void *dst, size_t dpitch, const void *src, size_t spitch,

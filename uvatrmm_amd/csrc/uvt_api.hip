@@ -298,6 +298,11 @@ extern "C" uvt_status uvt_destroy(uvt_handle h)
     if (!h) return UVT_ERR_ARG;
     (void)hipSetDevice(h->device);
     if (h->work) (void)hipFree(h->work);
+    for (int i = 0; i < 2; ++i) {
+        if (h->pin_done[i]) (void)hipEventSynchronize(h->pin_done[i]);
+        if (h->pin[i]) (void)hipHostFree(h->pin[i]);
+        if (h->pin_done[i]) (void)hipEventDestroy(h->pin_done[i]);
+    }
     free(h);
     return UVT_OK;
 }
@@ -804,6 +809,120 @@ extern "C" uvt_status uvt_free(uvt_handle h, void* dptr)
     return UVT_OK;
 }
 
+// Bytes of each pinned bounce buffer of host copies (two per handle), and
+// the size from which a contiguous host copy goes through them: staged
+// through pinned memory a copy runs at about 37 GB/s on MI355X, a pageable
+// one at about 27 GB/s.
+constexpr size_t kPinBytes = (size_t)16 << 20;
+constexpr size_t kPinMinBytes = (size_t)1 << 20;
+
+static uvt_status pin_setup(uvt_handle h)
+{
+    if (h->pin[0]) return UVT_OK;
+    for (int i = 0; i < 2; ++i) {
+        if (hipHostMalloc(&h->pin[i], kPinBytes, hipHostMallocDefault) != hipSuccess ||
+            hipEventCreateWithFlags(&h->pin_done[i], hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            for (int j = 0; j < 2; ++j) {
+                if (h->pin[j]) (void)hipHostFree(h->pin[j]);
+                if (h->pin_done[j]) (void)hipEventDestroy(h->pin_done[j]);
+                h->pin[j] = nullptr;
+                h->pin_done[j] = nullptr;
+            }
+            return UVT_ERR_NOMEM;
+        }
+        h->pin_used[i] = false;
+    }
+    return UVT_OK;
+}
+
+// Contiguous host <-> device copy through the handle's pinned buffers, a
+// chunk in flight while the host fills (or drains) the other.
+static uvt_status copy_pinned(uvt_handle h, char* dst, const char* src, size_t bytes, bool to_device)
+{
+    const size_t chunks = (bytes + kPinBytes - 1) / kPinBytes;
+    auto len = [&](size_t c) { return std::min(kPinBytes, bytes - c * kPinBytes); };
+    if (to_device) {
+        for (size_t c = 0; c < chunks; ++c) {
+            const int b = (int)(c & 1);
+            if (h->pin_used[b]) UVT_HIP(hipEventSynchronize(h->pin_done[b]));
+            memcpy(h->pin[b], src + c * kPinBytes, len(c));
+            UVT_HIP(hipMemcpyAsync(dst + c * kPinBytes, h->pin[b], len(c), hipMemcpyHostToDevice, 0));
+            UVT_HIP(hipEventRecord(h->pin_done[b], 0));
+            h->pin_used[b] = true;
+        }
+        UVT_HIP(hipStreamSynchronize(0));
+        return UVT_OK;
+    }
+    auto issue = [&](size_t c) -> hipError_t {
+        const int b = (int)(c & 1);
+        hipError_t e = hipMemcpyAsync(h->pin[b], src + c * kPinBytes, len(c), hipMemcpyDeviceToHost, 0);
+        if (e == hipSuccess) e = hipEventRecord(h->pin_done[b], 0);
+        h->pin_used[b] = true;
+        return e;
+    };
+    UVT_HIP(issue(0));
+    for (size_t c = 0; c < chunks; ++c) {
+        if (c + 1 < chunks) UVT_HIP(issue(c + 1));
+        const int b = (int)(c & 1);
+        UVT_HIP(hipEventSynchronize(h->pin_done[b]));
+        memcpy(dst + c * kPinBytes, h->pin[b], len(c));
+    }
+    return UVT_OK;
+}
+
+// Strided host <-> device copy through the handle's pinned buffers: rows are
+// packed (or unpacked) on the host a chunk at a time, and each chunk moves as
+// one rectangular copy on the null stream while the host packs the next.  A
+// pageable strided copy instead pays a fixed cost per row (about 9 us on
+// MI355X: 56 ms for the three 2048 x 2048 operand copies of one call).
+static uvt_status copy_2d_pinned(uvt_handle h, char* dst, size_t dpitch, const char* src, size_t spitch,
+                                 size_t width, size_t height, bool to_device)
+{
+    const size_t per = kPinBytes / width;
+    if (to_device) {
+        for (size_t r = 0, c = 0; r < height; r += per, ++c) {
+            const size_t rows = std::min(per, height - r);
+            const int b = (int)(c & 1);
+            char* buf = (char*)h->pin[b];
+            if (h->pin_used[b]) UVT_HIP(hipEventSynchronize(h->pin_done[b]));
+            for (size_t i = 0; i < rows; ++i) memcpy(buf + i * width, src + (r + i) * spitch, width);
+            if (dpitch == width)
+                UVT_HIP(hipMemcpyAsync(dst + r * dpitch, buf, rows * width, hipMemcpyHostToDevice, 0));
+            else
+                UVT_HIP(hipMemcpy2DAsync(dst + r * dpitch, dpitch, buf, width, width, rows,
+                                         hipMemcpyHostToDevice, 0));
+            UVT_HIP(hipEventRecord(h->pin_done[b], 0));
+            h->pin_used[b] = true;
+        }
+        UVT_HIP(hipStreamSynchronize(0));
+        return UVT_OK;
+    }
+    // device -> host: chunk c + 1 is in flight while chunk c is unpacked
+    auto issue = [&](size_t c) -> hipError_t {
+        const size_t r = c * per, rows = std::min(per, height - r);
+        const int b = (int)(c & 1);
+        hipError_t e = spitch == width
+                           ? hipMemcpyAsync(h->pin[b], src + r * spitch, rows * width, hipMemcpyDeviceToHost, 0)
+                           : hipMemcpy2DAsync(h->pin[b], width, src + r * spitch, spitch, width, rows,
+                                              hipMemcpyDeviceToHost, 0);
+        if (e == hipSuccess) e = hipEventRecord(h->pin_done[b], 0);
+        h->pin_used[b] = true;
+        return e;
+    };
+    const size_t chunks = (height + per - 1) / per;
+    UVT_HIP(issue(0));
+    for (size_t c = 0; c < chunks; ++c) {
+        if (c + 1 < chunks) UVT_HIP(issue(c + 1));
+        const int b = (int)(c & 1);
+        UVT_HIP(hipEventSynchronize(h->pin_done[b]));
+        const size_t r = c * per, rows = std::min(per, height - r);
+        const char* buf = (const char*)h->pin[b];
+        for (size_t i = 0; i < rows; ++i) memcpy(dst + (r + i) * dpitch, buf + i * width, width);
+    }
+    return UVT_OK;
+}
+
 extern "C" uvt_status uvt_memcpy(uvt_handle h, void* dst, const void* src, size_t bytes,
                                  int32_t kind)
 {
@@ -813,6 +932,11 @@ extern "C" uvt_status uvt_memcpy(uvt_handle h, void* dst, const void* src, size_
                                   : hipMemcpyDeviceToDevice;
     if (kind < 0 || kind > 2) return UVT_ERR_ARG;
     UVT_HIP(hipSetDevice(h->device));
+    if (kind != 2 && bytes >= kPinMinBytes) {
+        const uvt_status st = pin_setup(h);
+        if (st != UVT_OK) return st;
+        return copy_pinned(h, (char*)dst, (const char*)src, bytes, kind == 0);
+    }
     if (bytes) UVT_HIP(hipMemcpy(dst, src, bytes, k));
     return UVT_OK;
 }
@@ -827,6 +951,13 @@ extern "C" uvt_status uvt_memcpy_2d(uvt_handle h, void* dst, size_t dpitch, cons
                             : kind == 1 ? hipMemcpyDeviceToHost
                                         : hipMemcpyDeviceToDevice;
     UVT_HIP(hipSetDevice(h->device));
+    if (dpitch == width && spitch == width)  // one contiguous block
+        return uvt_memcpy(h, dst, src, width * height, kind);
+    if (kind != 2 && width <= kPinBytes && height > 1) {
+        const uvt_status st = pin_setup(h);
+        if (st != UVT_OK) return st;
+        return copy_2d_pinned(h, (char*)dst, dpitch, (const char*)src, spitch, width, height, kind == 0);
+    }
     UVT_HIP(hipMemcpy2D(dst, dpitch, src, spitch, width, height, k));
     return UVT_OK;
 }

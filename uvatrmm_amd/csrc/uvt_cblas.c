@@ -4,9 +4,11 @@
  * Per call: check the parameters (CBLAS numbering), reduce a column-major
  * call to the row-major one it equals (a column-major matrix is the
  * transpose of a row-major one), copy the operands into a grow-only device
- * staging buffer with even row pitches (so the tile kernel can use 16-byte
- * loads), run, and copy the result back.  One device handle per process,
- * created on the first call; a mutex serializes calls. */
+ * staging buffer, run, and copy the result back.  An operand whose rows lie
+ * back to back in host memory keeps that pitch on the device (one copy);
+ * others get an even pitch (16-byte loads in the tile kernel) and go through
+ * libuvt's pinned strided copy.  One device handle per process, created on
+ * the first call; a mutex serializes calls. */
 #include <pthread.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -33,6 +35,8 @@ static void bad_param(const char *fn, int pos)
 }
 
 static int64_t even(int64_t x) { return x + (x & 1); }
+/* device row pitch of a host operand of `cols` columns and leading dimension ld */
+static int64_t pitch(int64_t cols, int64_t ld) { return ld == cols ? cols : even(cols); }
 static size_t up256(size_t x) { return (x + 255) & ~(size_t)255; }
 static int64_t imax1(int64_t x) { return x > 1 ? x : 1; }
 
@@ -108,7 +112,7 @@ void cblas_dgemm(CBLAS_LAYOUT layout, CBLAS_TRANSPOSE transa, CBLAS_TRANSPOSE tr
     }
     const int read_ab = K > 0 && alpha != 0.0;
     const int64_t ra = tA ? K : M, ca = tA ? M : K, rb = tB ? N : K, cb = tB ? K : N;
-    const int64_t pa = even(imax1(ca)), pb = even(imax1(cb)), pc = even(N);
+    const int64_t pa = pitch(imax1(ca), la), pb = pitch(imax1(cb), lb), pc = pitch(N, ldc);
     const size_t abytes = read_ab ? up256((size_t)ra * pa * 8) : 0;
     const size_t bbytes = read_ab ? up256((size_t)rb * pb * 8) : 0;
     const size_t cbytes = up256((size_t)M * pc * 8);
@@ -156,7 +160,7 @@ void cblas_dtrmm(CBLAS_LAYOUT layout, CBLAS_SIDE side, CBLAS_UPLO uplo, CBLAS_TR
         N = m;
     }
     const int64_t nt = left ? M : N;
-    const int64_t pa = even(nt), pb = even(N);
+    const int64_t pa = pitch(nt, lda), pb = pitch(N, ldb);
     const size_t abytes = alpha != 0.0 ? up256((size_t)nt * pa * 8) : 0;
     const size_t bbytes = up256((size_t)M * pb * 8);
 

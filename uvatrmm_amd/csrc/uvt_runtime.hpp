@@ -16,6 +16,10 @@ struct uvt_handle_s {
     void* work;        // grows on demand (norm row sums, BLAS-form packing)
     size_t work_bytes;
     void* scratch;     // this device's fixed scratch array (uvt_dtrmm_noalloc)
+    // pinned bounce buffers of strided host copies (uvt_memcpy_2d), on first use
+    void* pin[2];
+    hipEvent_t pin_done[2];
+    bool pin_used[2];
 };
 
 constexpr int kMaxDevices = 64;
