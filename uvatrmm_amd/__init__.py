@@ -125,6 +125,7 @@ _SIGS = {
     "uvt_malloc": (i32, [vp, sz, _c.POINTER(vp)]),
     "uvt_free": (i32, [vp, vp]),
     "uvt_memcpy": (i32, [vp, vp, vp, sz, i32]),
+    "uvt_memcpy_2d": (i32, [vp, vp, sz, vp, sz, sz, sz, i32]),
     "uvt_memset": (i32, [vp, vp, i32, sz]),
     "uvt_stream_create": (i32, [vp, _c.POINTER(vp)]),
     "uvt_stream_destroy": (i32, [vp, vp]),
@@ -271,6 +272,22 @@ class DeviceBuffer:
             check(lib().uvt_memcpy(self.h.ptr, out[k].ctypes.data, self.ptr + int(r) * ld * isz,
                                    ncols * isz, 1), "d2h rows")
         return out
+
+    def upload_2d(self, host, width, height, dpitch, spitch):
+        """height rows of width bytes from host (byte pitch spitch) to this
+        buffer (byte pitch dpitch)."""
+        assert (height - 1) * dpitch + width <= self.nbytes
+        assert host.flags.c_contiguous and (height - 1) * spitch + width <= host.nbytes
+        check(lib().uvt_memcpy_2d(self.h.ptr, self.ptr, dpitch, host.ctypes.data, spitch, width,
+                                  height, 0), "h2d 2d")
+
+    def download_2d(self, host, width, height, dpitch, spitch):
+        """height rows of width bytes from this buffer (byte pitch spitch) into
+        host (byte pitch dpitch); the rest of host is left as it was."""
+        assert (height - 1) * spitch + width <= self.nbytes
+        assert host.flags.c_contiguous and (height - 1) * dpitch + width <= host.nbytes
+        check(lib().uvt_memcpy_2d(self.h.ptr, host.ctypes.data, dpitch, self.ptr, spitch, width,
+                                  height, 1), "d2h 2d")
 
     def zero(self):
         check(lib().uvt_memset(self.h.ptr, self.ptr, 0, self.nbytes))
