@@ -372,7 +372,10 @@ int blocks_per_cu(int32_t cfg)
 // -> 0.45 ms; at 4096 x 4096 (0.97x) cutting cost 7 %, between 0.9x and
 // 1.6x it changed -1 to +7 %, and cutting the boxes of the N=2048 boxes
 // plan (1.5x) cost 11 %, so boxes plans and plans below 1.5x keep whole
-// tiles (and the kernel without partial tiles).
+// tiles (and the kernel without partial tiles).  (A trapezoid plan of
+// 128x128 tiles whose whole tiles would not finish together runs stream-K
+// instead, decided before this in plan_upload; so cuts serve the smaller
+// tile shapes, where they were faster than stream-K.)
 void build_launch(const uvt_plan_s& pl, int cus, std::vector<LaunchTile> launch[2],
                   std::vector<PartSum>& sums, int64_t& nparts)
 {
