@@ -14,7 +14,9 @@ timed with HIP events on the stream the kernel runs on.
 Beside it: the split-precision leg on the same inputs (time, and the error
 statement of each call), the 512-problem batch (routed by measured speed,
 and with its split half forced onto the int8 path), the reference's other
-two configurations, and the served stream.
+two configurations, and the served stream (large requests first come first
+served; then the same stream with them shortest first, which lowers p99 and
+the mean while the largest requests wait longer: p999 and max beside it).
 cpu_baseline: the reference's per-rank stage loop (cblas_dgemm + cblas_dtrmm
 with MKL, sequential, one thread per rank) replayed on the host cores for a
 bounded column sample of the same matrices, median of 5 runs, for this
@@ -289,7 +291,7 @@ def batch_leg(args, h, uvt, route):
     return out
 
 
-def serve_leg(args, h, uvt):
+def serve_leg(args, h, uvt, big_wait_ms=0):
     """Served stream: TRMM requests with orders log-uniform in [128, 16384], B
     columns uniform in {64..4096}, half FP64 and half split class, inputs
     generated on the device per request, under a 200 GB HBM admission budget;
@@ -297,7 +299,8 @@ def serve_leg(args, h, uvt):
     Capacity is measured first (a burst of requests submitted at once); the
     stream then arrives as a Poisson process at `serve_load` x capacity, and
     latency is arrival (submit) -> result ready in HBM, over the requests
-    after the warm-up ones."""
+    after the warm-up ones.  big_wait_ms > 0: large requests held back by the
+    cap go shortest first, oldest first once they have waited that long."""
     rng = np.random.default_rng(args.seed + 29)
     table = os.path.join(ROOT, "uvatrmm_amd", "tune", "f64_tiles.json")
     table = table if os.path.exists(table) else None
@@ -307,7 +310,8 @@ def serve_leg(args, h, uvt):
 
     budget = int(args.serve_budget_gb * 1e9)
     srv = uvt.Server(h, budget, max_slots=args.serve_slots, slices=args.split_moduli or 13,
-                     table=table, big_flops=args.serve_big_flops, max_big=args.serve_max_big)
+                     table=table, big_flops=args.serve_big_flops, max_big=args.serve_max_big,
+                     big_wait_ms=big_wait_ms)
     seed = [args.seed * 1000003]
 
     def submit(req, tag):
@@ -363,13 +367,15 @@ def serve_leg(args, h, uvt):
         if sel:
             v = np.sort([d["t_done"] - d["t_submit"] for d in sel])
             classes[name] = {"requests": len(sel), "p50_ms": round(pct(v, 0.5) * 1e3, 4),
-                             "p99_ms": round(pct(v, 0.99) * 1e3, 4)}
+                             "p99_ms": round(pct(v, 0.99) * 1e3, 4),
+                             "max_ms": round(float(v[-1]) * 1e3, 4)}
     int8 = [d for d in meas if d["path"] == 1]
     span = max(d["t_done"] for d in meas) - min(d["t_submit"] for d in meas)
     fl = sum(float(stream[d["tag"] - 1_000_000][1]) * stream[d["tag"] - 1_000_000][0] ** 2
              for d in meas)
     out = {"requests": len(meas), "warmup_requests": args.serve_warmup,
            "p50_ms": round(rank(0.50) * 1e3, 4), "p99_ms": round(rank(0.99) * 1e3, 4),
+           "p999_ms": round(rank(0.999) * 1e3, 4),
            "mean_ms": round(float(lat.mean()) * 1e3, 4), "max_ms": round(float(lat[-1]) * 1e3, 4),
            "throughput_tflops": round(fl / span * 1e-12, 4),
            "requests_per_s": round(len(meas) / span, 2),
@@ -377,6 +383,8 @@ def serve_leg(args, h, uvt):
            "capacity_per_s": round(capacity, 2), "capacity_tflops": round(burst_tflops, 4),
            "slots": args.serve_slots, "budget_gb": args.serve_budget_gb,
            "large_request_flops": args.serve_big_flops, "max_large_in_flight": args.serve_max_big,
+           "large_order": ("first come first served" if big_wait_ms == 0 else
+                           f"shortest first; oldest first once waited {big_wait_ms} ms"),
            "peak_arena_gb": round(st["peak_bytes"] / 1e9, 3),
            "peak_inflight": st["peak_inflight"],
            "split_class_share": round(float(np.mean([p for _, _, p in stream])), 3),
@@ -503,6 +511,9 @@ def main():
     ap.add_argument("--serve-max-big", type=int, default=1,
                     help="large requests in flight at once (0 = no cap)")
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-alt-wait-ms", type=int, default=150,
+                    help="second serve leg on the same stream with the large requests shortest "
+                         "first, except those waiting this many ms or longer (0 = skip it)")
     ap.add_argument("--no-vendor-ceiling", dest="vendor_ceiling", action="store_false",
                     help="skip timing the vendor int8 and FP64 GEMMs (torch) at n=16384")
     ap.add_argument("--vendor-only", action="store_true", help=argparse.SUPPRESS)
@@ -568,6 +579,10 @@ def main():
     other = configs_leg(args, h, uvt) if not args.no_configs else None
 
     serve = serve_leg(args, h, uvt) if args.serve_requests > 0 else None
+    # the same stream with the large requests shortest first (a lower p99 and
+    # mean for a longer wait of the largest requests: p999 and max reported)
+    serve_sf = (serve_leg(args, h, uvt, args.serve_alt_wait_ms)
+                if args.serve_requests > 0 and args.serve_alt_wait_ms > 0 else None)
     vendor = None
     if args.vendor_ceiling:
         # a child process: torch brings its own HIP runtime, which does not
@@ -650,6 +665,7 @@ def main():
         "batch_forced_split": batch_forced,
         "other_configs": other,
         "serve": serve,
+        "serve_shortest_first": serve_sf,
         "vendor_gemm": vendor,
         "norm": norm,
     }

@@ -298,7 +298,10 @@ uvt_status uvt_tune_choice(const char *tune_table, int64_t n, int64_t m, int32_t
  * of m n^2 >= big_flops run at once, and smaller requests queued behind a
  * held-back large one are admitted past it (large ones keep their order).
  * Sharing the GPU among fewer large requests shortens each of them, which
- * is what sets the tail latency of a mixed stream. */
+ * is what sets the tail latency of a mixed stream.  With big_wait_ms > 0 the
+ * large requests go shortest first, which lowers the latency of most of
+ * them, and the longest then wait up to about big_wait_ms plus the work
+ * ahead of them. */
 typedef struct {
     size_t hbm_budget; /* bytes of the operand arena (the slots' partial tiles come on top) */
     int32_t max_slots; /* requests in flight on the device (1..64) */
@@ -306,7 +309,9 @@ typedef struct {
     const char *tune_table; /* FP64 tile shapes (NULL: built-in) */
     double big_flops;  /* m n^2 from which a request counts as large */
     int32_t max_big;   /* large requests in flight at once (0: no cap) */
-    int32_t reserved;  /* 0 */
+    int32_t big_wait_ms; /* order of large requests held by the cap: 0 first come first
+                            served; > 0 the one of fewest flops first, except that one
+                            waiting this many ms or longer goes first (oldest first) */
 } uvt_serve_config;
 
 typedef struct {

@@ -4,6 +4,7 @@ else is in flight (bit for bit equal to the same request served alone), the
 arena never holds more than the budget, oversize requests are refused, and
 every result agrees with the FP64 oracle within its class's tolerance."""
 import sys
+import time
 
 import numpy as np
 import pytest
@@ -218,8 +219,49 @@ def test_serve_large_request_cap(uvt, gpu, oracle):
         assert rel_fro(outs[k], ref) <= 1e-13
 
 
+@pytest.mark.parametrize("wait_ms", [10000, 1])
+def test_serve_large_shortest_first(uvt, gpu, wait_ms):
+    """With big_wait_ms, the large requests held back by the cap go shortest
+    first (big_wait_ms = 10 s: none waits that long here), or oldest first
+    once they have waited big_wait_ms (1 ms: all of them, while the first
+    large request runs for about 4 ms); results bit for bit as served alone."""
+    reqs = [(8192, 4096, 0, 7101), (1400, 700, 1, 7102), (300, 50, 0, 7103), (1600, 500, 0, 7104),
+            (1500, 560, 0, 7105), (1300, 800, 1, 7106), (250, 30, 0, 7107)]
+    big = 5e8  # large: m n^2 = 2.7e11, 1.37e9, 1.28e9, 1.26e9, 1.35e9
+    srv = uvt.Server(gpu, 8 * GiB, max_slots=8, slices=13, big_flops=big, max_big=1,
+                     big_wait_ms=wait_ms)
+    outs = [np.full((n, m), np.nan) for n, m, _, _ in reqs]
+    try:
+        # the first large request alone (the scheduler admits it at once),
+        # then the others while it runs
+        srv.submit_seeded(*reqs[0][:3], reqs[0][3], spread=1, C=outs[0], tag=0)
+        time.sleep(0.001)
+        for k, (n, m, prec, seed) in enumerate(reqs[1:], 1):
+            srv.submit_seeded(n, m, prec, seed, spread=1, C=outs[k], tag=k)
+        srv.drain()
+        done = {d["tag"]: d for d in srv.poll()}
+    finally:
+        srv.close()
+    assert sorted(done) == list(range(len(reqs))) and all(d["status"] == uvt.OK for d in done.values())
+    flops = {k: m * n * n for k, (n, m, _, _) in enumerate(reqs)}
+    large = [d["tag"] for d in sorted((done[k] for k in flops if flops[k] >= big), key=lambda d: d["t_admit"])]
+    times = {k: (round(d["t_submit"] * 1e3, 3), round(d["t_admit"] * 1e3, 3), round(d["t_done"] * 1e3, 3))
+             for k, d in done.items()}
+    say("large admitted in order", large, "wait_ms", wait_ms, "times", times)
+    # the scenario: the others were all queued while the first one ran
+    assert large[0] == 0 and all(done[k]["t_submit"] < done[0]["t_done"] for k in large), times
+    rest = large[1:]
+    if wait_ms > 100:
+        assert rest == sorted(rest, key=lambda k: flops[k]), large
+    else:
+        assert rest == sorted(rest), large
+    alone, _, _ = serve(uvt, gpu, reqs, 8 * GiB, 1)
+    for k, (n, m, prec, seed) in enumerate(reqs):
+        assert np.array_equal(alone[k], outs[k]), (k, n, m)
+
+
 def test_serve_config_checks(uvt, gpu):
-    for kw in ({"max_big": -1}, {"big_flops": -1.0}):
+    for kw in ({"max_big": -1}, {"big_flops": -1.0}, {"big_wait_ms": -1}):
         with pytest.raises(uvt.UvtError) as e:
             uvt.Server(gpu, 1 << 30, max_slots=2, **kw)
         assert e.value.status == uvt.ERR_ARG

@@ -44,7 +44,7 @@ class Problem(_c.Structure):
 
 class ServeConfig(_c.Structure):
     _fields_ = [("hbm_budget", sz), ("max_slots", i32), ("slices", i32), ("tune_table", _c.c_char_p),
-                ("big_flops", f64), ("max_big", i32), ("reserved", i32)]
+                ("big_flops", f64), ("max_big", i32), ("big_wait_ms", i32)]
 
 
 class Request(_c.Structure):
@@ -446,10 +446,10 @@ class Server:
     """Continuous-batching TRMM server (see uvt_serve_create)."""
 
     def __init__(self, handle, hbm_budget, max_slots=8, slices=13, table=None, big_flops=0.0,
-                 max_big=0):
+                 max_big=0, big_wait_ms=0):
         self.h = handle
         cfg = ServeConfig(hbm_budget, max_slots, slices, table.encode() if table else None,
-                          big_flops, max_big, 0)
+                          big_flops, max_big, big_wait_ms)
         p = vp()
         check(lib().uvt_serve_create(handle.ptr, _c.byref(cfg), _c.byref(p)), "serve_create")
         self.ptr = p

@@ -210,6 +210,26 @@ static bool is_big(const uvt_serve_config& cfg, const uvt_request& r)
     return cfg.max_big > 0 && (double)r.m * (double)r.n * (double)r.n >= cfg.big_flops;
 }
 
+// The large request to admit next when they go shortest first: the oldest
+// one that has waited big_wait_ms or longer, else the one of fewest flops
+// (the earliest among equals).  Some large request is pending.
+static std::deque<Pending>::iterator next_big(uvt_server s)
+{
+    const double now = s->now(), wait = s->cfg.big_wait_ms * 1e-3;
+    auto best = s->pending.end();
+    double best_flops = 0.0;
+    for (auto it = s->pending.begin(); it != s->pending.end(); ++it) {
+        if (!is_big(s->cfg, it->r)) continue;
+        if (now - it->t_submit >= wait) return it;  // the queue is in arrival order
+        const double f = (double)it->r.m * (double)it->r.n * (double)it->r.n;
+        if (best == s->pending.end() || f < best_flops) {
+            best = it;
+            best_flops = f;
+        }
+    }
+    return best;
+}
+
 // A free slot on the least busy hardware queue.  HIP spreads streams over
 // its hardware queues in turn (GPU_MAX_HW_QUEUES of them, 4 by default), so
 // slot i shares a queue with slots i +- hwq; a kernel queued behind another
@@ -297,14 +317,15 @@ static void scheduler(uvt_server s)
         }
         if (progress) s->cv_drain.notify_all();
         // 2. admission: FIFO, except that a large request held back by the
-        // cap lets later requests pass (large ones keep their order); a
-        // request that does not fit in the arena stops admission, so memory
-        // is granted in arrival order
+        // cap lets later requests pass (large ones keep their order, or go
+        // shortest first with big_wait_ms); a request that does not fit in the
+        // arena stops admission, so memory is granted in queue order
         for (;;) {
             auto it = s->pending.begin();
             while (it != s->pending.end() && is_big(s->cfg, it->r) && s->big_inflight >= s->cfg.max_big)
                 ++it;
             if (it == s->pending.end()) break;
+            if (is_big(s->cfg, it->r) && s->cfg.big_wait_ms > 0) it = next_big(s);
             Slot* free_slot = pick_slot(s);
             if (!free_slot) break;
             // one contiguous block per request; none free yet: wait for
@@ -346,7 +367,7 @@ extern "C" uvt_status uvt_serve_create(uvt_handle h, const uvt_serve_config* cfg
 {
     if (!h || !cfg || !out || cfg->max_slots < 1 || cfg->max_slots > 64 || cfg->slices < 2 ||
         cfg->slices > 14 || cfg->hbm_budget < 4096 || !(cfg->big_flops >= 0.0) || cfg->max_big < 0 ||
-        cfg->reserved != 0)
+        cfg->big_wait_ms < 0)
         return UVT_ERR_ARG;
     std::vector<uvt_tune_entry> tune;
     if (cfg->tune_table && !uvt_tune_load(cfg->tune_table, tune)) return UVT_ERR_ARG;
