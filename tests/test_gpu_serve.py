@@ -219,15 +219,20 @@ def test_serve_large_request_cap(uvt, gpu, oracle):
         assert rel_fro(outs[k], ref) <= 1e-13
 
 
-@pytest.mark.parametrize("wait_ms", [10000, 1])
-def test_serve_large_shortest_first(uvt, gpu, wait_ms):
-    """With big_wait_ms, the large requests held back by the cap go shortest
-    first (big_wait_ms = 10 s: none waits that long here), or oldest first
-    once they have waited big_wait_ms (1 ms: all of them, while the first
-    large request runs for about 4 ms); results bit for bit as served alone."""
-    reqs = [(8192, 4096, 0, 7101), (1400, 700, 1, 7102), (300, 50, 0, 7103), (1600, 500, 0, 7104),
-            (1500, 560, 0, 7105), (1300, 800, 1, 7106), (250, 30, 0, 7107)]
-    big = 5e8  # large: m n^2 = 2.7e11, 1.37e9, 1.28e9, 1.26e9, 1.35e9
+@pytest.mark.parametrize("wait_ms,expect", [
+    (0, "arrival"),       # first come first served
+    (10000, "size"),      # shortest first; none waits 10 s here
+    (1, "arrival"),       # shortest first, but all have waited 1 ms: oldest first
+])
+def test_serve_large_order(uvt, gpu, wait_ms, expect):
+    """big_wait_ms: which large request held back by the cap goes next.  A
+    large request keeps the GPU busy for about 4 ms while four more, their
+    m n^2 3x apart, arrive largest first; they are then admitted in arrival
+    order or by size as big_wait_ms says.  Results bit for bit as served
+    alone."""
+    reqs = [(8192, 4096, 0, 7101), (3000, 3000, 0, 7102), (2000, 2250, 1, 7103), (1440, 1450, 0, 7104),
+            (1000, 1000, 1, 7105), (300, 50, 0, 7106)]
+    big = 5e8  # large: m n^2 = 2.7e11, then 2.7e10, 9.0e9, 3.0e9, 1.0e9; the last one is small
     srv = uvt.Server(gpu, 8 * GiB, max_slots=8, slices=13, big_flops=big, max_big=1,
                      big_wait_ms=wait_ms)
     outs = [np.full((n, m), np.nan) for n, m, _, _ in reqs]
@@ -250,11 +255,8 @@ def test_serve_large_shortest_first(uvt, gpu, wait_ms):
     say("large admitted in order", large, "wait_ms", wait_ms, "times", times)
     # the scenario: the others were all queued while the first one ran
     assert large[0] == 0 and all(done[k]["t_submit"] < done[0]["t_done"] for k in large), times
-    rest = large[1:]
-    if wait_ms > 100:
-        assert rest == sorted(rest, key=lambda k: flops[k]), large
-    else:
-        assert rest == sorted(rest), large
+    want = [1, 2, 3, 4] if expect == "arrival" else [4, 3, 2, 1]
+    assert large[1:] == want, (large, times)
     alone, _, _ = serve(uvt, gpu, reqs, 8 * GiB, 1)
     for k, (n, m, prec, seed) in enumerate(reqs):
         assert np.array_equal(alone[k], outs[k]), (k, n, m)
