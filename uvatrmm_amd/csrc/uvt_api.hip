@@ -605,6 +605,10 @@ int64_t uvt_sk_workgroups(int32_t cfg, int64_t n, int64_t m, int cus)
     const int64_t tiles = (int64_t)g.nrt * g.nct;
     if (tiles > 16 * slots || g.G < slots) return 0;  // many tiles balance by themselves
     const int64_t even = (g.P + slots - 1) / slots;
+    // List scheduling never takes longer than an even share plus the deepest
+    // tile (Graham 1969), so with a deepest tile below (kSkGain - 1) x the
+    // share the span cannot reach kSkGain x: no need to simulate it.
+    if ((double)sk_depth(g, g.nrt - 1) < (kSkGain - 1.0) * (double)even) return 0;
     return (double)whole_tile_panels(g, slots) >= kSkGain * (double)even ? slots : 0;
 }
 
