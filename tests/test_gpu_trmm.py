@@ -392,3 +392,45 @@ def test_max_order_65536(gpu, uvt, oracle):
     ref = oracle.seeded_rows_dd(n, m, 91, 1, rows)
     for r, i in enumerate(rows):
         assert rel_fro(C[i], ref[r]) <= TOL, i
+
+
+def test_random_plans_exact(gpu, uvt):
+    """A seeded sweep of plans: orders log-uniform in [1, 4500], B widths in
+    [1, 2500], both mappings and shapes, then deep narrow trapezoids (n in
+    [5000, 9000], m in [100, 1300]); 1..16 bands, odd and even leading
+    dimensions.  Small-integer inputs make every product exact in any
+    summation order, so each result must equal NumPy's bit for bit, whatever
+    the plan launches on this GPU (whole tiles, cut reductions, stream-K,
+    boxes).  Plans the mapping refuses (a band without rows) are skipped."""
+    rng = np.random.default_rng(2026)
+    seen = set()
+    for case in range(42):
+        if case < 36:
+            n = int(np.exp(rng.uniform(0, np.log(4500))))
+            m = int(np.exp(rng.uniform(0, np.log(2500))))
+            shape = ("trapezoid", "boxes")[(case // 2) % 2]
+        else:  # deep and narrow trapezoids: their reductions are split
+            n = int(rng.integers(5000, 9001))
+            m = int(rng.integers(100, 1301))
+            shape = "trapezoid"
+        p = int(rng.integers(1, min(16, n) + 1))
+        mapping = ("reg", "balanced")[case % 2]
+        pad = int(rng.integers(0, 2))
+        try:
+            plan = uvt.Plan(n, m, p, mapping, shape)
+        except uvt.UvtError as e:
+            assert e.status == uvt.ERR_MAPPING
+            continue
+        wg, parts = gpu.plan_launch_info(plan)
+        plan.close()
+        how = "whole" if parts == 0 else ("stream-k" if wg == 2 * gpu.cus and shape == "trapezoid" else "cut")
+        seen.add(how)
+        say("random plan", case, (n, m, p, mapping, shape, pad), how, wg, parts)
+        A = np.tril(rng.integers(-2, 3, (n, n)).astype(np.float64))
+        B = rng.integers(-2, 3, (n, m)).astype(np.float64)
+        Ap = np.zeros((n, n + pad))
+        Ap[:, :n] = A
+        C = run_trmm(gpu, uvt, Ap, B, p, mapping, shape, lda=n + pad)
+        np.testing.assert_array_equal(C, A @ B, err_msg=str((n, m, p, mapping, shape, pad)))
+    say("launch forms seen", sorted(seen))
+    assert "whole" in seen and len(seen) >= 2
