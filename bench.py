@@ -43,13 +43,14 @@ sys.path.insert(0, ROOT)
 from uvatrmm_amd import workloads  # noqa: E402  (seeded workload definitions)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix peak
-# HBM bytes per launch of the FP64 tile kernel at N=32768, M=8192 (balanced,
-# trapezoid, 16 bands), from two rocprofv3 counter runs of this script:
-# 2.0 x FETCH_SIZE (92,351,490 KiB; the factor is what FETCH_SIZE reads for
-# k_row_abs_sums streaming exactly 2 GiB in 128-byte requests, the request
-# size of the tile kernel's 128-byte row segments) + WRITE_SIZE (2,097,152
-# KiB, exactly C).  Kept per workload; other workloads report null.
-MEASURED_TRAFFIC = {(32768, 8192, 16, "balanced", "trapezoid"): 2.0 * 92351490 * 1024 + 2097152 * 1024}
+# HBM bytes per launch of the FP64 tile kernel (256x128 tiles) at N=32768,
+# M=8192 (balanced, trapezoid, 16 bands), from two rocprofv3 counter runs of
+# this script: 2.0 x FETCH_SIZE (58,506,032 KiB; the factor is what
+# FETCH_SIZE reads for k_row_abs_sums streaming exactly 2 GiB in 128-byte
+# requests, the request size of the tile kernel's 128-byte row segments) +
+# WRITE_SIZE (2,097,152 KiB, exactly C).  Kept per workload; other workloads
+# report null.
+MEASURED_TRAFFIC = {(32768, 8192, 16, "balanced", "trapezoid"): 2.0 * 58506032 * 1024 + 2097152 * 1024}
 
 
 def log(*a):
@@ -625,7 +626,7 @@ def main():
     wg, parts = h.plan_launch_info(plan)
     cfg0 = int(plan.tiles(0)["cfg"][0]) if len(plan.tiles(0)) else 0
     tile, waves = {0: ("128,128", "8 waves of 32x64"), 1: ("64,64", "4 waves of 32x32"),
-                   2: ("128,32", "4 waves of 32x32")}[cfg0]
+                   2: ("128,32", "4 waves of 32x32"), 4: ("256,128", "16 waves of 32x64")}[cfg0]
     vec = "true" if n % 2 == 0 and m % 2 == 0 else "false"
     launches = plan.phases
     if parts and cfg0 == 0 and plan.phases == 1:

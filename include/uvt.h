@@ -95,10 +95,13 @@ uvt_status uvt_plan_bands(uint64_t n, uint32_t p, uvt_map map, uvt_range *out, i
 typedef struct uvt_plan_s *uvt_plan;
 
 /* One output tile of the FP64 kernels: rows [r0, r0+rows), columns
- * [c0, c0+tile_n) clipped to m, reduction over k in [k_lo, k_hi). */
+ * [c0, c0+tile_n) clipped to m, reduction over k in [k_lo, k_hi).  cfg is
+ * its shape: a uvt_tile_cfg, or UVT_TILE_256x128 in the plans of large
+ * problems (at least 1024 such tiles). */
 typedef struct {
     int32_t r0, rows, c0, k_lo, k_hi, flags, band, cfg;
 } uvt_tile;
+#define UVT_TILE_256x128 4 /* 256 x 128 tiles (not selectable per batch problem) */
 #define UVT_TILE_LOWER 1 /* A[i][k] is treated as 0 for k > i */
 #define UVT_TILE_ACCUM 2 /* C += product (else C = product) */
 

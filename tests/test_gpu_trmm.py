@@ -276,7 +276,7 @@ def test_headline_plan_keeps_whole_tiles(gpu, uvt):
     plan = uvt.Plan(32768, 8192, 16, "balanced", "trapezoid")
     wg, parts = gpu.plan_launch_info(plan)
     plan.close()
-    assert (wg, parts) == (256 * 64, 0)
+    assert (wg, parts) == (128 * 64, 0)  # 256x128 tiles, whole
 
 
 def test_large_row_sampled(gpu, uvt, oracle):

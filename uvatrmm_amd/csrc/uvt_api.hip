@@ -356,6 +356,7 @@ constexpr double kSplitMinFlops = 2e9;
 
 int blocks_per_cu(int32_t cfg)
 {
+    if (cfg == UVT_CFG_256x128) return TileWaves<256, 128>::BLOCKS_PER_CU;
     if (cfg == UVT_CFG_64x64) return TileWaves<64, 64>::BLOCKS_PER_CU;
     if (cfg == UVT_CFG_128x32) return TileWaves<128, 32>::BLOCKS_PER_CU;
     return TileWaves<128, 128>::BLOCKS_PER_CU;
@@ -522,7 +523,9 @@ static uvt_status launch_tiles_f64(int32_t cfg, const void* dtiles, int64_t ntil
     if (ntiles > 0x7FFFFFFF) return UVT_ERR_SIZE;
     const bool vec = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && lda % 2 == 0 &&
                      ldb % 2 == 0;
-    if (cfg == UVT_CFG_64x64)
+    if (cfg == UVT_CFG_256x128)
+        launch_tiles<256, 128>(dtiles, ntiles, vec, A, lda, B, ldb, C, ldc, m, parts, s);
+    else if (cfg == UVT_CFG_64x64)
         launch_tiles<64, 64>(dtiles, ntiles, vec, A, lda, B, ldb, C, ldc, m, parts, s);
     else if (cfg == UVT_CFG_128x32)
         launch_tiles<128, 32>(dtiles, ntiles, vec, A, lda, B, ldb, C, ldc, m, parts, s);
@@ -627,6 +630,7 @@ uvt_status uvt_launch_trapezoid_sk_f64(int32_t cfg, int64_t n, int64_t m, const 
                                        int64_t workgroups, double* parts, hipStream_t s)
 {
     if (n <= 0 || m <= 0 || lda < n || ldb < m || ldc < m || workgroups <= 0 || !parts) return UVT_ERR_ARG;
+    if (cfg == UVT_CFG_256x128) return UVT_ERR_ARG;  // stream-K runs 128x128 tiles (uvt_sk_workgroups)
     if (n > 0x3FFFFFFF || m > 0x3FFFFFFF || workgroups > 0x7FFFFFFF) return UVT_ERR_SIZE;
     const SkGeo g = sk_geometry(n, m, uvt_cfg_tile_m(cfg), uvt_cfg_tile_n(cfg), workgroups);
     const bool vec = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && lda % 2 == 0 &&
@@ -648,7 +652,8 @@ uvt_status uvt_launch_trapezoid_f64(int32_t cfg, int64_t n, int64_t m, const dou
     if (((n + tm - 1) / tm) * ((m + tn - 1) / tn) > 0x7FFFFFFF) return UVT_ERR_SIZE;
     const bool vec = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && lda % 2 == 0 &&
                      ldb % 2 == 0;
-    if (cfg == UVT_CFG_64x64) launch_trapezoid<64, 64>(vec, n, m, A, lda, B, ldb, C, ldc, s);
+    if (cfg == UVT_CFG_256x128) launch_trapezoid<256, 128>(vec, n, m, A, lda, B, ldb, C, ldc, s);
+    else if (cfg == UVT_CFG_64x64) launch_trapezoid<64, 64>(vec, n, m, A, lda, B, ldb, C, ldc, s);
     else if (cfg == UVT_CFG_128x32) launch_trapezoid<128, 32>(vec, n, m, A, lda, B, ldb, C, ldc, s);
     else launch_trapezoid<128, 128>(vec, n, m, A, lda, B, ldb, C, ldc, s);
     UVT_HIP(hipGetLastError());

@@ -335,7 +335,7 @@ uvt_status dtrmm_launch(const DtrmmCall& c, int64_t m, int64_t n, double alpha, 
         wp += c.x_bytes;
     }
     double* R = (double*)wp;
-    uvt_status st = uvt_launch_trapezoid_f64(uvt_default_cfg(c.nt, c.w), c.nt, c.w, L, la, X, lxx,
+    uvt_status st = uvt_launch_trapezoid_f64(uvt_wide_tiles(uvt_default_cfg(c.nt, c.w), c.nt, c.w), c.nt, c.w, L, la, X, lxx,
                                              R, c.lr, s);
     if (st != UVT_OK) return st;
     // B <- alpha * unpack(R): B(i, j) = R(rmap(i), j) (left) or R(cmap(j), i) (right)

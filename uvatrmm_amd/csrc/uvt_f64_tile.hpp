@@ -312,6 +312,9 @@ template <int TM, int TN> struct TileWaves;
 template <> struct TileWaves<128, 128> {
     static constexpr int M = 4, N = 2, BLOCKS_PER_CU = 2, WAVES_PER_EU = 4;
 };
+template <> struct TileWaves<256, 128> {
+    static constexpr int M = 8, N = 2, BLOCKS_PER_CU = 1, WAVES_PER_EU = 4;
+};
 template <> struct TileWaves<64, 64> {
     static constexpr int M = 2, N = 2, BLOCKS_PER_CU = 3, WAVES_PER_EU = 3;
 };

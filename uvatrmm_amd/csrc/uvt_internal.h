@@ -19,6 +19,10 @@ enum { UVT_CFG_128x128 = 0, UVT_CFG_64x64 = 1, UVT_CFG_128x32 = 2, UVT_NUM_CFGS 
 /* 128x64 tiles: only the last column block of a batch problem whose columns
  * end 33..64 past a multiple of 128 (not a tuning choice). */
 enum { UVT_CFG_128x64 = 3 };
+/* 256x128 tiles (16 waves, one workgroup per CU): the tile walks of large
+ * problems only (plans, the server's and the BLAS trapezoid), never a batch
+ * or the tuner (uvt_wide_tiles). */
+enum { UVT_CFG_256x128 = 4 };
 UVT_INTERNAL int32_t uvt_cfg_tile_m(int32_t cfg);
 UVT_INTERNAL int32_t uvt_cfg_tile_n(int32_t cfg);
 
@@ -52,6 +56,9 @@ struct uvt_plan_s {
 UVT_INTERNAL int64_t uvt_plan_trapezoid_tiles(int64_t n, int64_t m, int32_t cfg, uvt_tile *out);
 /* Built-in tile-shape choice for an n x n by n x m product. */
 UVT_INTERNAL int32_t uvt_default_cfg(int64_t n, int64_t m);
+/* The shape a tile walk of that product runs: cfg, or 256x128 for a
+ * 128x128 problem of at least 1024 such tiles. */
+UVT_INTERNAL int32_t uvt_wide_tiles(int32_t cfg, int64_t n, int64_t m);
 
 /* Releases device copies of a plan (defined on the HIP side). */
 UVT_INTERNAL void uvt_plan_release_device(uvt_plan plan);

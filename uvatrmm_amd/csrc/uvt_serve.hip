@@ -195,7 +195,7 @@ static uvt_status launch(uvt_server s, Slot& sl)
     if (int8) {
         e = uvt_trmm_split(s->h, n, m, s->cfg.slices, dA, n, dB, m, dC, m, work, wbytes, st);
     } else {
-        const int tile = s->tune.empty() ? uvt_default_cfg(n, m) : uvt_tune_pick(s->tune, n, m);
+        const int tile = uvt_wide_tiles(s->tune.empty() ? uvt_default_cfg(n, m) : uvt_tune_pick(s->tune, n, m), n, m);
         const int64_t sk = uvt_sk_workgroups(tile, n, m, s->h->cus);
         e = sk > 0 ? uvt_launch_trapezoid_sk_f64(tile, n, m, dA, n, dB, m, dC, m, sk, sl.parts, st)
                    : uvt_launch_trapezoid_f64(tile, n, m, dA, n, dB, m, dC, m, st);
