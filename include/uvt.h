@@ -97,7 +97,8 @@ typedef struct uvt_plan_s *uvt_plan;
 /* One output tile of the FP64 kernels: rows [r0, r0+rows), columns
  * [c0, c0+tile_n) clipped to m, reduction over k in [k_lo, k_hi).  cfg is
  * its shape: a uvt_tile_cfg, or UVT_TILE_256x128 in the plans of large
- * problems (at least 1024 such tiles). */
+ * problems whose 256 x 128 tiles are all full (n a multiple of 256, m of
+ * 128, at least 512 tiles). */
 typedef struct {
     int32_t r0, rows, c0, k_lo, k_hi, flags, band, cfg;
 } uvt_tile;

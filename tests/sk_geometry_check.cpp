@@ -127,7 +127,7 @@ static void check(int64_t n, int64_t m, int tm, int tn, int64_t G0)
 
 int main()
 {
-    const int shapes[][2] = {{128, 128}, {64, 64}, {128, 32}};
+    const int shapes[][2] = {{128, 128}, {256, 128}, {64, 64}, {128, 32}};
     const int64_t ns[] = {1, 15, 16, 17, 64, 127, 128, 129, 300, 1000, 2048, 5000, 8165, 16384};
     const int64_t ms[] = {1, 31, 32, 33, 128, 200, 939, 2000};
     const int64_t gs[] = {1, 2, 7, 512, 768, 2048, 1 << 30};

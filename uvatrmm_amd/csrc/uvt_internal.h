@@ -57,7 +57,7 @@ UVT_INTERNAL int64_t uvt_plan_trapezoid_tiles(int64_t n, int64_t m, int32_t cfg,
 /* Built-in tile-shape choice for an n x n by n x m product. */
 UVT_INTERNAL int32_t uvt_default_cfg(int64_t n, int64_t m);
 /* The shape a tile walk of that product runs: cfg, or 256x128 for a
- * 128x128 problem of at least 1024 such tiles. */
+ * 128x128 problem whose 256x128 tiles are all full and at least 512. */
 UVT_INTERNAL int32_t uvt_wide_tiles(int32_t cfg, int64_t n, int64_t m);
 
 /* Releases device copies of a plan (defined on the HIP side). */

@@ -52,14 +52,20 @@ int32_t uvt_default_cfg(int64_t n, int64_t m)
     return UVT_CFG_128x128;
 }
 
-/* A large walk moves a quarter fewer operand bytes per flop in 256x128
- * tiles (16 waves of 32x64, one workgroup per CU, results bit for bit those
- * of 128x128 tiles): measured on MI355X 0.4-0.6 % faster at N=32768,
- * M=8192 and 1.0-1.7 % at 8192^2, 16384 x 4096 and 16384^2.  From 1024
- * such tiles (four per CU), so that they still balance. */
+/* A walk of 256x128 tiles (16 waves of 32x64, one workgroup per CU; results
+ * bit for bit those of 128x128 tiles) moves a quarter fewer operand bytes
+ * per flop.  Measured on MI355X against 128x128 tiles, same box: where every
+ * tile is full (n a multiple of 256, m of 128) 0.4-0.6 % faster at N=32768,
+ * M=8192, 0.3-1.7 % at 8192^2, 0.8-1.4 % at 16384 x 4096, 1.0-1.7 % at
+ * 16384^2, 0.8 % at 8192 x 4096 and 3-4 % at 4096^2, 8192 x 2048 and
+ * 16384 x 1024; with edge tiles 1-5 % slower in six of eight shapes (the
+ * idle waves of an edge tile leave half of a one-workgroup CU empty, where
+ * two 128x128 workgroups fill each other's gaps).  So: full tiles only, at
+ * least two per CU (512 on MI355X). */
 int32_t uvt_wide_tiles(int32_t cfg, int64_t n, int64_t m)
 {
-    if (cfg == UVT_CFG_128x128 && ((n + 255) / 256) * ((m + 127) / 128) >= 4 * 256) return UVT_CFG_256x128;
+    if (cfg == UVT_CFG_128x128 && n % 256 == 0 && m % 128 == 0 && (n / 256) * (m / 128) >= 2 * 256)
+        return UVT_CFG_256x128;
     return cfg;
 }
 
