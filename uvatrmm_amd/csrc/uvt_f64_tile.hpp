@@ -14,8 +14,8 @@
 // v_mfma_f64_16x16x4_f64 (one double of A and B per lane, four doubles of
 // C per lane).  A keeps its row-major layout in LDS (rows of BK + 2 doubles)
 // and B its k-major one, so every LDS write is one 16-byte piece a thread
-// loaded; the writes for the next panel go out after the second of the four
-// MFMA steps of the current one, and each lane reads its A values for a
+// loaded; the writes for the next panel are issued during the current
+// panel's products (see compute()), and each lane reads its A values for a
 // panel as 16-byte pairs of one row.  (A transposed to k-major on the way in took
 // four conflicting 8-byte writes per thread and cost about 10 % of the MFMA
 // rate, measured register-only.)
@@ -183,11 +183,16 @@ __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int
     };
 
     // The products of the panel in LDS buffer buf; with st, the registers
-    // loaded for the next panel go to buffer buf ^ 1 after the second of the
-    // four MFMA steps, so the writes complete under the last two instead of
-    // in front of the barrier.  (Measured at N=32768, M=8192: 69.5-70.4
-    // TFLOP/s against 67.4-69.2 with the writes after the first, third or
-    // fourth step.)
+    // loaded for the next panel go to buffer buf ^ 1.  The source places the
+    // writes after the second of the four MFMA steps (measured at N=32768,
+    // M=8192: 69.5-70.4 TFLOP/s against 67.4-69.2 with them after the
+    // first, third or fourth step); the compiler's scheduler then issues them
+    // after the first 4 products of the panel (256x128 tiles) or 8 (128x128),
+    // and moves the barrier that follows ahead of the last step's products.
+    // Pinning the writes with __builtin_amdgcn_sched_barrier after 0, 2, 4,
+    // 8, 16 or 24 products measured 3-12 % slower (4 included: the pins also
+    // stop the other interleaving), as did the max-ilp and iterative-ilp
+    // scheduler strategies (0.8-1.2 %); same bits in every case.
     //
     // k order inside a panel: at MFMA step s, lane group g = lane / 16 takes
     // k = g * BK/4 + s (A and B alike, so the four steps still cover the
