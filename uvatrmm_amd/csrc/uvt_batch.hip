@@ -87,16 +87,16 @@ void k_batch_f64(const BatchTile* __restrict__ tiles, int ntiles, const BatchPro
         // it keeps each shape's offset arithmetic inside its own branch
         const int tid = (int)threadIdx.x + p.pad;
         if (bt.cfg == UVT_CFG_64x64)
-            f64_tile<64, 64, VEC, 2, 4>(bt.r0, bt.rows, bt.c0, bt.k_lo, bt.k_hi, kTileLower, gbl(p.A),
+            f64_tile<64, 64, VEC, 2, 4, 16, true, true>(bt.r0, bt.rows, bt.c0, bt.k_lo, bt.k_hi, kTileLower, gbl(p.A),
                                         p.lda, gbl(p.B), p.ldb, gbl(p.C), p.ldc, p.m, smem, tid);
         else if (bt.cfg == UVT_CFG_128x64)
-            f64_tile<128, 64, VEC, 4, 2>(bt.r0, bt.rows, bt.c0, bt.k_lo, bt.k_hi, kTileLower, gbl(p.A),
+            f64_tile<128, 64, VEC, 4, 2, 16, true, true>(bt.r0, bt.rows, bt.c0, bt.k_lo, bt.k_hi, kTileLower, gbl(p.A),
                                          p.lda, gbl(p.B), p.ldb, gbl(p.C), p.ldc, p.m, smem, tid);
         else if (bt.cfg == UVT_CFG_128x32)
-            f64_tile<128, 32, VEC, 4, 2>(bt.r0, bt.rows, bt.c0, bt.k_lo, bt.k_hi, kTileLower, gbl(p.A),
+            f64_tile<128, 32, VEC, 4, 2, 16, true, true>(bt.r0, bt.rows, bt.c0, bt.k_lo, bt.k_hi, kTileLower, gbl(p.A),
                                          p.lda, gbl(p.B), p.ldb, gbl(p.C), p.ldc, p.m, smem, tid);
         else
-            f64_tile<128, 128, VEC, 4, 2>(bt.r0, bt.rows, bt.c0, bt.k_lo, bt.k_hi, kTileLower, gbl(p.A),
+            f64_tile<128, 128, VEC, 4, 2, 16, true, true>(bt.r0, bt.rows, bt.c0, bt.k_lo, bt.k_hi, kTileLower, gbl(p.A),
                                           p.lda, gbl(p.B), p.ldb, gbl(p.C), p.ldc, p.m, smem, tid);
         __syncthreads();  // the LDS tile is reused by the next tile
     }

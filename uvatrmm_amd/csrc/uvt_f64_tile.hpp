@@ -9,7 +9,7 @@
 //   * the trapezoid C[r0:r0+TM] = A[r0:, 0:r0+TM] * B        (mask, store)
 //
 // Data path: A and B k-panels (BK = 16 deep) stream from HBM with 16-byte
-// loads into registers one panel ahead, are written to a double-buffered LDS
+// loads into registers one panel ahead (two with AHEAD2), are written to a double-buffered LDS
 // tile, and each wave multiplies its (TM/WAVES_M) x (TN/WAVES_N) sub-tile with
 // v_mfma_f64_16x16x4_f64 (one double of A and B per lane, four doubles of
 // C per lane).  A keeps its row-major layout in LDS (rows of BK + 2 doubles)
@@ -82,7 +82,15 @@ struct F64Cfg {
 // EDGE_IDLE: waves of an edge tile that hold no row or column of C run no
 // products (off where a caller's own registers leave no room for the test:
 // the stream-K kernel then spills inside its masked k loop).
-template <int TM, int TN, bool VEC, int WAVES_M = 2, int WAVES_N = 2, int BK_ = 16, bool EDGE_IDLE = true>
+// AHEAD2: the global loads of panel kt + 2 are issued right after panel
+// kt + 1 goes to LDS (during the products of panel kt) instead of at the top
+// of panel kt + 1, so the load registers are never live across a panel's
+// start.  It lowers register pressure where one kernel holds several tile
+// shapes (the persistent batch kernel: 12 bytes of spills per lane instead
+// of 52, 1.5-3 % faster); the single-shape kernels ran 0.2-1.7 % slower
+// with it.  The sums and their order are the same either way.
+template <int TM, int TN, bool VEC, int WAVES_M = 2, int WAVES_N = 2, int BK_ = 16, bool EDGE_IDLE = true,
+          bool AHEAD2 = false>
 __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int k_hi, int flags,
                                          const UVT_GLOBAL double* __restrict__ A, int64_t lda,
                                          const UVT_GLOBAL double* __restrict__ B, int64_t ldb,
@@ -201,7 +209,7 @@ __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int
     // pairs: half the LDS reads of one 8-byte read per step (1.4 % on a
     // register-only probe of this loop, 73.2 -> 74.3 TFLOP/s).
     static_assert(BK % 8 == 0, "panel depth: whole 16-byte A pairs per lane group");
-    auto compute = [&](int buf, bool st) {
+    auto compute = [&](int buf, bool st, auto&& next) {
         const int kg = (lane >> 4) * (BK / 4);
         const double* as = As + buf * G::A_ELEMS + (wm * G::WM + (lane & 15)) * G::SA + kg;
         const double* bs = Bs + buf * G::B_ELEMS + kg * G::LDB_S + wn * G::WN + (lane & 15);
@@ -222,7 +230,10 @@ __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int
 #pragma unroll
                     for (int j = 0; j < G::FN; ++j)
                         acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2[i][h], b[j], acc[i][j], 0, 0, 0);
-                if (s == 1 && st) store(buf ^ 1);
+                if (s == 1 && st) {
+                    store(buf ^ 1);
+                    next();  // AHEAD2: the loads of the panel after next
+                }
             }
         }
     };
@@ -241,12 +252,24 @@ __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int
         if (kt0 == 0 && nfull > 0) load(k_start, full);
         else load(k_start, masked);
         store(0);
+        if (AHEAD2 && nk > 1) {
+            if (1 < nfull) load(k_start + BK, full);  // panel 1 is full from kt0 <= 1 on
+            else load(k_start + BK, masked);
+        }
         __syncthreads();
         int kt = 0;
-        for (; kt + 1 < nfull; ++kt) {  // the next panel is full
-            load(k_start + (kt + 1) * BK, full);
-            compute(kt & 1, true);
-            __syncthreads();
+        // Registers hold the next panel's values from here on when AHEAD2.
+        if constexpr (AHEAD2) {
+            for (; kt + 2 < nfull; ++kt) {  // the panel after next is full
+                compute(kt & 1, true, [&] { load(k_start + (kt + 2) * BK, full); });
+                __syncthreads();
+            }
+        } else {
+            for (; kt + 1 < nfull; ++kt) {  // the next panel is full
+                load(k_start + (kt + 1) * BK, full);
+                compute(kt & 1, true, [] {});
+                __syncthreads();
+            }
         }
         // In a lower tile, panels that start past the last of this wave's
         // rows meet only masked (zero) A values: the wave runs them without
@@ -267,14 +290,17 @@ __device__ __forceinline__ void f64_tile(int r0, int rows, int c0, int k_lo, int
         }
         for (; kt < nk_w; ++kt) {
             const bool more = kt + 1 < nk;
-            if (more) load(k_start + (kt + 1) * BK, masked);
-            compute(kt & 1, more);
+            if (!AHEAD2 && more) load(k_start + (kt + 1) * BK, masked);
+            compute(kt & 1, more, [&] {
+                if (AHEAD2 && kt + 2 < nk) load(k_start + (kt + 2) * BK, masked);
+            });
             __syncthreads();
         }
         for (; kt < nk; ++kt) {
             if (kt + 1 < nk) {
-                load(k_start + (kt + 1) * BK, masked);
+                if (!AHEAD2) load(k_start + (kt + 1) * BK, masked);
                 store((kt & 1) ^ 1);
+                if (AHEAD2 && kt + 2 < nk) load(k_start + (kt + 2) * BK, masked);
             }
             __syncthreads();
         }
