@@ -431,7 +431,7 @@ int orc_stage_loop(int64_t n, int64_t m, uint32_t p, int balanced, const double 
     double *scratch = (double *)malloc((size_t)(maxh ? maxh : 1) * (size_t)m * sizeof(double));
     if (!scratch) return -2;
     for (uint32_t s = 0; s < p; s++) {
-        uint64_t b, h;
+        uint64_t b = 0, h = 0; /* every band mapped without error above */
         if (balanced)
             orc_map_balanced((uint64_t)n, p, s, &b, &h);
         else
