@@ -343,6 +343,8 @@ template <int TM, int TN> struct TileWaves;
 template <> struct TileWaves<128, 128> {
     static constexpr int M = 4, N = 2, BLOCKS_PER_CU = 2, WAVES_PER_EU = 4;
 };
+// 256x128 tiles: 16 waves of 32x64; as 8 waves of 64x64 (a third fewer LDS
+// reads, two waves per SIMD, 234 VGPRs) they measured 11-13 % slower.
 template <> struct TileWaves<256, 128> {
     static constexpr int M = 8, N = 2, BLOCKS_PER_CU = 1, WAVES_PER_EU = 4;
 };
