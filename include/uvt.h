@@ -144,16 +144,17 @@ int32_t uvt_device_cus(uvt_handle h);
  * reductions and adds the pieces after the first of a tile to C afterwards
  * in k order (results repeat bit for bit), with up to 64 MiB of partial
  * tiles on the device: with 128x128 tiles whose whole-tile span is 1.3x or
- * more an even share of the work, stream-K (one workgroup per slot, each the
- * same number of k panels); with smaller tiles whose deepest tile is 1.5x or
+ * more an even share of the work, stream-K (up to one workgroup per slot,
+ * each the same number of k panels, in groups that run a row tile's column
+ * tiles side by side); with smaller tiles whose deepest tile is 1.5x or
  * more a slot's share, cuts of those tiles into pieces.  The device state
  * belongs to the plan: calls with one plan on different streams must be
  * serialized. */
 uvt_status uvt_trmm_f64(uvt_handle h, uvt_plan plan, const double *dA, int64_t lda,
                         const double *dB, int64_t ldb, double *dC, int64_t ldc, void *stream);
 /* How the plan launches on the handle's device (placing its device copies
- * there if needed): workgroups of its tile launches (for stream-K, one per
- * workgroup slot), and partial tiles (0 when every tile writes C directly).
+ * there if needed): workgroups of its tile launches (for stream-K, up to one
+ * per workgroup slot), and partial tiles (0 when every tile writes C directly).
  * Either pointer may be NULL. */
 uvt_status uvt_plan_launch_info(uvt_handle h, uvt_plan plan, int64_t *workgroups, int64_t *partial_tiles);
 

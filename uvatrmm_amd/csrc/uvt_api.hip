@@ -95,10 +95,11 @@ void k_trmm_trapezoid_f64(int64_t n, int64_t m, int nrt, int nct, const double* 
                                       gbl(B), ldb, gbl(C), ldc, m, smem);
 }
 
-// The trapezoid walk split stream-K (uvt_sk.hpp): workgroup b runs the
-// pieces of panels [b P / G, (b+1) P / G) one after another; its first piece
-// writes partial tile b if it starts inside its tile, every other piece
-// starts at k = 0 and writes C.
+// The trapezoid walk split stream-K (uvt_sk.hpp): workgroup w of group
+// b = w / ncg runs the pieces of panels [b P / G, (b+1) P / G) one after
+// another, in column tile w % ncg of each piece's row tile when the launch
+// runs column groups; its first piece writes partial tile w if it starts
+// inside its tile, every other piece starts at k = 0 and writes C.
 template <int TM, int TN, bool VEC>
 __global__ __launch_bounds__((64 * TileWaves<TM, TN>::M * TileWaves<TM, TN>::N),
                              (TileWaves<TM, TN>::WAVES_PER_EU))
@@ -115,14 +116,15 @@ void k_trmm_sk_f64(SkGeo g, const double* __restrict__ A, int64_t lda, const dou
     auto uni64 = [&](int64_t v) {
         return (int64_t)(((uint64_t)(uint32_t)uni((int)(v >> 32)) << 32) | (uint32_t)uni((int)v));
     };
-    // Workgroup i runs on XCD i % 8: the 8 XCDs take contiguous eighths of
-    // the sequence, so the workgroups that share an XCD's L2 work on
-    // neighbouring tiles (the same rows of A at about the same k).
-    const int64_t i = blockIdx.x;
-    const int64_t b = g.G % kXcds == 0 ? (i % kXcds) * (g.G / kXcds) + i / kXcds : i;
+    // The 8 XCDs take contiguous runs of the launch, so the workgroups that
+    // share an XCD's L2 work on neighbouring tiles, or (column groups) on the
+    // same rows of A at the same k.
+    const int64_t w = sk_xcd_order(blockIdx.x, sk_launch(g));
+    const int64_t b = w / g.ncg;
+    const int cx = (int)(w % g.ncg) * TN;  // the member's column tile (0 without groups)
     int64_t x = uni64(sk_start(g, b));
     const int64_t xe = uni64(sk_start(g, b + 1));
-    UVT_GLOBAL double* part = gbl(parts) + b * (TM * TN);
+    UVT_GLOBAL double* part = gbl(parts) + w * (TM * TN);
     bool first = true;
     while (x < xe) {
         const SkPiece s = sk_piece(g, x, xe);
@@ -132,7 +134,7 @@ void k_trmm_sk_f64(SkGeo g, const double* __restrict__ A, int64_t lda, const dou
         // the piece loop
         int tid = (int)threadIdx.x;
         asm volatile("" : "+v"(tid));
-        f64_tile<TM, TN, VEC, W::M, W::N, kSkPanel, false>(uni(s.r0), uni(s.rows), uni(s.c0), uni(s.k_lo), uni(s.k_hi),
+        f64_tile<TM, TN, VEC, W::M, W::N, kSkPanel, false>(uni(s.r0), uni(s.rows), uni(s.c0 + cx), uni(s.k_lo), uni(s.k_hi),
                                           kTileLower, gbl(A), lda, gbl(B), ldb, gbl(C), ldc, g.m, smem,
                                           tid, uni(s.p0 > 0) ? part : nullptr);
         first = false;
@@ -142,13 +144,15 @@ void k_trmm_sk_f64(SkGeo g, const double* __restrict__ A, int64_t lda, const dou
 
 // After k_trmm_sk_f64: for each tile split between workgroups, the
 // workgroup b whose first piece is the tile's first partial tile adds
-// partial tiles b, b+1, ... (k order) to C, which holds the k = 0 piece.
-// (gridDim.y workgroups per split tile, each over its share of the rows.)
+// partial tiles b, b+1, ... (k order) to C, which holds the k = 0 piece; with
+// column groups, group b's member in each column tile does so in that column
+// tile.  (gridDim.y workgroups per split tile, each over its share of the rows.)
 template <int TM, int TN>
 __global__ __launch_bounds__(256) void k_sk_fixup(SkGeo g, const double* __restrict__ parts,
                                                   double* __restrict__ C, int64_t ldc)
 {
-    const int64_t b = blockIdx.x;
+    const int64_t w = blockIdx.x;  // partial tile w: group b, column tile cx
+    const int64_t b = w / g.ncg, cx = w % g.ncg;
     if (b == 0) return;
     const int64_t x = sk_start(g, b);
     const int j = sk_row(g, x);
@@ -159,8 +163,9 @@ __global__ __launch_bounds__(256) void k_sk_fixup(SkGeo g, const double* __restr
     int64_t cnt = 1;
     while (b + cnt < g.G && sk_start(g, b + cnt) < tile0 + d) ++cnt;
     const int r0 = j * TM, rows = (int)(g.n - r0 < TM ? g.n - r0 : TM);
-    const int64_t c0 = (off / d) * TN;
-    const UVT_GLOBAL double* P = gbl(parts) + b * (TM * TN);
+    const int64_t c0 = (off / d + cx) * TN;
+    const UVT_GLOBAL double* P = gbl(parts) + w * (TM * TN);
+    const int64_t step = g.ncg * (int64_t)(TM * TN);  // group b + q's partial tile in this column tile
     const int per = (rows + (int)gridDim.y - 1) / (int)gridDim.y;
     const int e_lo = (int)blockIdx.y * per * TN;
     const int e_hi = min(rows, ((int)blockIdx.y + 1) * per) * TN;
@@ -169,7 +174,7 @@ __global__ __launch_bounds__(256) void k_sk_fixup(SkGeo g, const double* __restr
         if (c0 + c >= g.m) continue;
         UVT_GLOBAL double* p = gbl(C) + (int64_t)(r0 + r) * ldc + (c0 + c);
         double v = *p;
-        for (int64_t q = 0; q < cnt; ++q) v += P[q * (TM * TN) + e];
+        for (int64_t q = 0; q < cnt; ++q) v += P[q * step + e];
         *p = v;
     }
 }
@@ -353,6 +358,11 @@ constexpr size_t kMaxPartBytes = (size_t)64 << 20;  // partial tiles of one plan
 // Below this many flops a plan's launch time, not one tile's depth, bounds
 // it, and the plan keeps whole tiles.
 constexpr double kSplitMinFlops = 2e9;
+// Stream-K walks run column groups (uvt_sk.hpp) where there are at least 8 of
+// them.  Measured against one sequence over every tile (three alternating
+// passes, one box): 8192 x 1024 1.20 -> 1.18 ms, 8165 x 939 1.30 -> 1.29 ms,
+// 6772 x 1548 and 5000 x 2000 within 1 % either way.
+constexpr bool kSkGroupCols = true;
 
 int blocks_per_cu(int32_t cfg)
 {
@@ -445,9 +455,10 @@ static uvt_status plan_upload(uvt_handle h, uvt_plan plan)
         }
         plan->dev_id = h->device;
         plan->sk_grid = sk;
-        const SkGeo g = sk_geometry(plan->n, plan->m, uvt_cfg_tile_m(plan->cfg), uvt_cfg_tile_n(plan->cfg), sk);
+        const SkGeo g =
+            sk_geometry(plan->n, plan->m, uvt_cfg_tile_m(plan->cfg), uvt_cfg_tile_n(plan->cfg), sk, kSkGroupCols);
         for (int64_t b = 1; b < g.G; ++b)
-            if (sk_piece(g, sk_start(g, b), sk_start(g, b + 1)).p0 > 0) ++plan->nparts;
+            if (sk_piece(g, sk_start(g, b), sk_start(g, b + 1)).p0 > 0) plan->nparts += g.ncg;
         return UVT_OK;
     }
     std::vector<LaunchTile> launch[2];
@@ -553,14 +564,14 @@ template <int TM, int TN>
 static void launch_trapezoid_sk(bool vec, const SkGeo& g, const double* A, int64_t lda, const double* B,
                                 int64_t ldb, double* C, int64_t ldc, double* parts, hipStream_t s)
 {
-    const dim3 grid((unsigned)g.G), block(64 * TileWaves<TM, TN>::M * TileWaves<TM, TN>::N);
+    const dim3 grid((unsigned)sk_launch(g)), block(64 * TileWaves<TM, TN>::M * TileWaves<TM, TN>::N);
     if (vec)
         hipLaunchKernelGGL((k_trmm_sk_f64<TM, TN, true>), grid, block, 0, s, g, A, lda, B, ldb, C, ldc,
                            parts);
     else
         hipLaunchKernelGGL((k_trmm_sk_f64<TM, TN, false>), grid, block, 0, s, g, A, lda, B, ldb, C, ldc,
                            parts);
-    hipLaunchKernelGGL((k_sk_fixup<TM, TN>), dim3((unsigned)g.G, 4), dim3(256), 0, s, g, parts, C, ldc);
+    hipLaunchKernelGGL((k_sk_fixup<TM, TN>), dim3((unsigned)sk_launch(g), 4), dim3(256), 0, s, g, parts, C, ldc);
 }
 
 namespace {
@@ -632,7 +643,7 @@ uvt_status uvt_launch_trapezoid_sk_f64(int32_t cfg, int64_t n, int64_t m, const 
     if (n <= 0 || m <= 0 || lda < n || ldb < m || ldc < m || workgroups <= 0 || !parts) return UVT_ERR_ARG;
     if (cfg == UVT_CFG_256x128) return UVT_ERR_ARG;  // stream-K runs 128x128 tiles (uvt_sk_workgroups)
     if (n > 0x3FFFFFFF || m > 0x3FFFFFFF || workgroups > 0x7FFFFFFF) return UVT_ERR_SIZE;
-    const SkGeo g = sk_geometry(n, m, uvt_cfg_tile_m(cfg), uvt_cfg_tile_n(cfg), workgroups);
+    const SkGeo g = sk_geometry(n, m, uvt_cfg_tile_m(cfg), uvt_cfg_tile_n(cfg), workgroups, kSkGroupCols);
     const bool vec = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) && lda % 2 == 0 &&
                      ldb % 2 == 0;
     if (cfg == UVT_CFG_64x64) launch_trapezoid_sk<64, 64>(vec, g, A, lda, B, ldb, C, ldc, parts, s);
@@ -697,7 +708,11 @@ extern "C" uvt_status uvt_plan_launch_info(uvt_handle h, uvt_plan plan, int64_t*
     UVT_HIP(hipSetDevice(h->device));
     const uvt_status st = plan_upload(h, plan);
     if (st != UVT_OK) return st;
-    if (workgroups) *workgroups = plan->sk_grid > 0 ? plan->sk_grid : plan->nlaunch[0] + plan->nlaunch[1];
+    if (workgroups)
+        *workgroups = plan->sk_grid > 0 ? sk_launch(sk_geometry(plan->n, plan->m, uvt_cfg_tile_m(plan->cfg),
+                                                                uvt_cfg_tile_n(plan->cfg), plan->sk_grid,
+                                                                kSkGroupCols))
+                                        : plan->nlaunch[0] + plan->nlaunch[1];
     if (partial_tiles) *partial_tiles = plan->nparts;
     return UVT_OK;
 }

@@ -46,7 +46,8 @@ UVT_INTERNAL uvt_status uvt_handle_workspace(uvt_handle h, size_t bytes, void** 
 UVT_INTERNAL uvt_status uvt_launch_trapezoid_f64(int32_t cfg, int64_t n, int64_t m, const double* A, int64_t lda,
                                     const double* B, int64_t ldb, double* C, int64_t ldc,
                                     hipStream_t s);
-// The same walk split stream-K (uvt_sk.hpp) over `workgroups` workgroups, then
+// The same walk split stream-K (uvt_sk.hpp) over at most `workgroups`
+// workgroups (in column groups where there are at least 8 of them), then
 // its fixup launch; parts holds uvt_sk_parts_bytes(cfg, workgroups) bytes.
 // Each result is the same on every run with the same arguments and GPU.
 UVT_INTERNAL uvt_status uvt_launch_trapezoid_sk_f64(int32_t cfg, int64_t n, int64_t m, const double* A,

@@ -12,6 +12,13 @@
 // launch whose tiles differ in depth (narrow B: few, deep tiles) finishes
 // together instead of waiting for its deepest tile.
 //
+// Column groups (ncg > 1): for narrow B the nct column tiles of a row tile
+// share every A panel, so instead of laying them end to end the sequence
+// holds each row tile's panels once (nct = 1 in the arithmetic below) and a
+// launch of G groups x ncg workgroups runs it: member c of group b runs group
+// b's share of the sequence in column tile c.  The ncg members of a group
+// read the same A panels at about the same time, from one XCD's L2.
+//
 // Where a tile's panels are split between workgroups:
 //   - the piece that starts at k = 0 writes C;
 //   - every other piece is the first piece of its workgroup b (a workgroup's
@@ -29,10 +36,24 @@ constexpr int kXcds = 8;      // MI355X: workgroup i of a launch runs on XCD i %
 
 struct SkGeo {
     int64_t n, m;  // C = tril(A) * B: A n x n, B and C n x m
-    int64_t P;     // panels of the whole walk
+    int64_t P;     // panels of the whole walk (of one column tile when ncg > 1)
     int32_t nrt, nct, tm, tn;
-    int64_t G;     // workgroups, 1 <= G <= P
+    int64_t G;     // workgroups (groups when ncg > 1), 1 <= G <= P
+    int32_t ncg;   // column tiles per group: 1 (none), or every column tile
 };
+
+// Workgroups of the launch.
+__host__ __device__ inline int64_t sk_launch(const SkGeo& g) { return g.G * g.ncg; }
+
+// The launch index w that hardware workgroup i runs (workgroup i of a launch
+// runs on XCD i % 8): XCD x takes a contiguous run of w, so consecutive w --
+// the members of a group, and neighbouring shares of the sequence -- share
+// an XCD's L2.  A permutation of [0, nw).
+__host__ __device__ inline int64_t sk_xcd_order(int64_t i, int64_t nw)
+{
+    const int64_t x = i % kXcds, q = nw / kXcds, r = nw % kXcds;
+    return x * q + (x < r ? x : r) + i / kXcds;
+}
 
 // Panels before row tile j, 0 <= j <= nrt - 1 (row tiles before the last
 // are full: (r + 1) * tm / 16 panels each).
@@ -89,9 +110,11 @@ __host__ __device__ inline SkPiece sk_piece(const SkGeo& g, int64_t x, int64_t x
     return s;
 }
 
-// The geometry of an n x m walk in tm x tn tiles over G workgroups (G is
-// capped at P so that every workgroup has at least one panel).
-inline SkGeo sk_geometry(int64_t n, int64_t m, int tm, int tn, int64_t G)
+// The geometry of an n x m walk in tm x tn tiles over at most W workgroups
+// (every workgroup gets at least one panel).  group_cols: column groups
+// where there are at least kXcds of them (G = W / nct groups of nct), else
+// none.
+inline SkGeo sk_geometry(int64_t n, int64_t m, int tm, int tn, int64_t W, bool group_cols = false)
 {
     SkGeo g;
     g.n = n;
@@ -99,9 +122,13 @@ inline SkGeo sk_geometry(int64_t n, int64_t m, int tm, int tn, int64_t G)
     g.tm = tm;
     g.tn = tn;
     g.nrt = (int32_t)((n + tm - 1) / tm);
-    g.nct = (int32_t)((m + tn - 1) / tn);
+    const int32_t nct = (int32_t)((m + tn - 1) / tn);
+    const bool grouped = group_cols && nct > 1 && W / nct >= kXcds;
+    g.nct = grouped ? 1 : nct;
+    g.ncg = grouped ? nct : 1;
     g.G = 1;
     g.P = sk_before(g, g.nrt - 1) + (int64_t)g.nct * sk_depth(g, g.nrt - 1);
+    const int64_t G = grouped ? W / nct : W;
     g.G = G < 1 ? 1 : (G > g.P ? g.P : G);
     return g;
 }
