@@ -586,6 +586,7 @@ namespace {
 // plan path is faster).  More workgroups than slots (2x, 4x) did not help.
 constexpr double kSkMinFlops = 2e9;  // below: launch time, not balance, bounds a walk
 constexpr double kSkGain = 1.3;      // whole-tile span over an even share that stream-K needs
+                                     // (with column groups: 1-14 % slower up to 1.23, 2 % faster at 1.29)
 
 // How long whole tiles take, in panels per workgroup slot: the launch hands
 // tiles, deepest first, to the slot that frees first (list scheduling).
