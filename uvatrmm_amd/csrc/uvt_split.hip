@@ -253,7 +253,17 @@ __global__ __launch_bounds__(256) void k_alpha(Desc one, const Desc* table, cons
     const int64_t kb = row_box_end(row);
     const UVT_GLOBAL double* a = gbl(d.A) + row * d.lda;
     NormAcc box, dg;
-    for (int64_t k = lane; k < kb; k += 64) box.add(a[k]);
+    // Four loads in flight per lane, then the adds in the same k order as one
+    // at a time (the sums, and so the scales, do not change).
+    int64_t k = lane;
+    for (; k + 192 < kb; k += 256) {
+        const double v0 = a[k], v1 = a[k + 64], v2 = a[k + 128], v3 = a[k + 192];
+        box.add(v0);
+        box.add(v1);
+        box.add(v2);
+        box.add(v3);
+    }
+    for (; k < kb; k += 64) box.add(a[k]);
     for (int64_t k = kb + lane; k <= row; k += 64) dg.add(a[k]);
     box.wave_reduce();
     dg.wave_reduce();
@@ -284,7 +294,17 @@ __global__ __launch_bounds__(256) void k_beta_part(Desc one, const Desc* table, 
     const int64_t k0 = chunk * 256;
     const int64_t k1 = k0 + 256 < d.n ? k0 + 256 : d.n;
     NormAcc acc;
-    for (int64_t k = k0; k < k1; ++k) acc.add(gbl(d.B)[k * d.ldb + j]);
+    const UVT_GLOBAL double* b = gbl(d.B) + j;
+    int64_t k = k0;
+    for (; k + 3 < k1; k += 4) {  // four loads in flight, added in k order
+        const double v0 = b[k * d.ldb], v1 = b[(k + 1) * d.ldb], v2 = b[(k + 2) * d.ldb],
+                     v3 = b[(k + 3) * d.ldb];
+        acc.add(v0);
+        acc.add(v1);
+        acc.add(v2);
+        acc.add(v3);
+    }
+    for (; k < k1; ++k) acc.add(b[k * d.ldb]);
     const xd s2 = acc.sum_sq(), s1 = acc.sum_abs();
     d.part[chunk * d.m + j] = Part{s2.f, acc.nonfinite ? nan_value() : s1.f, s2.e, s1.e};
 }
